@@ -1,0 +1,169 @@
+#!/usr/bin/env python
+"""Benchmark: orbit RK-steps/s for 1,048,576 Milky Way orbits in FP64.
+
+Workload (one "step" = one pass of the hot path over the whole batch):
+1,048,576 orbits in MilkyWayPotential, R ~ U(0.5, 50) kpc, 25% velocity
+dispersion, per-orbit integration time log-uniform in 0.1..5 Gyr, adaptive
+Dopri8 (rtol 1e-8, atol 1e-10), 64 save times per orbit, one persistent
+launch with a longest-first work queue.  Inputs are resident in HBM before
+the timed region.  Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+
+def cpu_baseline(pot, solver, w0, T, nsave, out_gpu, seconds: float, threads: int):
+    """The oracle on a bounded sample of the same workload (a seeded random
+    subset of the orbits, integrated in chunks until `seconds` elapse); also
+    compares those orbits with the GPU's output."""
+    import numpy as np
+
+    import oracle
+
+    rng = np.random.default_rng(12345)
+    idx = rng.permutation(w0.shape[0])
+    steps = 0
+    done = 0
+    worst = 0.0
+    t_start = time.perf_counter()
+    chunk = 4 * threads
+    while time.perf_counter() - t_start < seconds and done < w0.shape[0]:
+        sel = idx[done : done + chunk]
+        out, status, nsteps, _ = oracle.integrate(pot, solver, w0[sel], 0.0, T[sel], T=nsave, nthreads=threads)
+        steps += int(nsteps.sum())
+        done += len(sel)
+        g = out_gpu[sel]
+        rel = np.linalg.norm(g - out, axis=-1) / np.linalg.norm(out, axis=-1)
+        worst = max(worst, float(np.nanmax(rel)))
+    elapsed = time.perf_counter() - t_start
+    return steps / elapsed, done, elapsed, worst
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_048_576)
+    ap.add_argument("--saves", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            f"bench.py measures one GPU (got --gpus {args.gpus}); independent orbits shard "
+            "across GPUs without communication, so run it once per GPU",
+            file=sys.stderr,
+        )
+        return 2
+
+    import numpy as np
+    import torch
+
+    import galax_amd.dynamics as gd
+    import galax_amd.potential as gp
+    from galax_amd import _lib, flops, workloads
+
+    torch.cuda.set_device(0)
+    pot = gp.MilkyWayPotential()
+    n = args.n
+    w0, t0, T = workloads.config_uneven(n=n, seed=2)
+    order = workloads.cost_order(w0, T)
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.PIDController(rtol=1e-8, atol=1e-10), max_steps=None)
+
+    w0d = torch.as_tensor(w0, device="cuda")
+    t0d = torch.as_tensor(t0, device="cuda")
+    Td = torch.as_tensor(T, device="cuda")
+    od = torch.as_tensor(order, device="cuda")
+    out = torch.empty((n, args.saves, 6), dtype=torch.float64, device="cuda")
+    ws = torch.empty((int(_lib.lib().gx_integrate_workspace_bytes(n)) + 7) // 8, dtype=torch.float64, device="cuda")
+    pot._handle()
+
+    def run():
+        return gd.integrate_orbits(pot, w0d, t0d, Td, T=args.saves, solver=solver, order=od, out=out, workspace=ws)
+
+    for _ in range(args.warmup):
+        res = run()
+    torch.cuda.synchronize()
+    print(f"[bench] warmup done, timing {args.steps} passes over {n} orbits", file=sys.stderr, flush=True)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    ev0.record()
+    for _ in range(args.steps):
+        res = run()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t_wall) / args.steps
+    ms = ev0.elapsed_time(ev1) / args.steps
+    status = res.status.cpu().numpy()
+    nsteps = res.nsteps.cpu().numpy()
+    naccept = res.naccept.cpu().numpy()
+    total_steps = int(nsteps.sum())
+    value = total_steps / (ms * 1e-3)
+    n_mini = n * max(args.saves - 2, 0)  # interior saves: one method step each
+    work = flops.f_step("dopri8") * total_steps + flops.f_save("dopri8") * n_mini
+    achieved = work / (ms * 1e-3) / 1e12
+    line = {
+        "metric": "orbit RK-steps/s (attempted Dopri8 steps, 1M Milky Way orbits, FP64)",
+        "value": value,
+        "unit": "orbit-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded PCG64 initial conditions)",
+        "config": {
+            "workload": "batched uneven exact FP64: 1,048,576 MW orbits, R~U(0.5,50) kpc, 25% dispersion, "
+            "T_i log-uniform 0.1-5 Gyr, adaptive Dopri8 rtol 1e-8 atol 1e-10, 64 saves per orbit",
+            "orbits": n,
+            "saves_per_orbit": args.saves,
+            "attempted_steps": total_steps,
+            "accepted_steps": int(naccept.sum()),
+            "failed_orbits": int((status != 0).sum()),
+            "wall_ms_per_step": wall * 1e3,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "pipe": "fp64 vector ALU (same 78.6 TFLOP/s peak as FP64 matrix)",
+            "achieved": achieved,
+            "peak": flops.PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / flops.PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "flop_per_step": flops.f_step("dopri8"),
+            "flop_per_save": flops.f_save("dopri8"),
+        },
+    }
+    if not args.no_cpu:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        out_np = out.cpu().numpy()
+        cps, done, el, worst = cpu_baseline(pot, solver, w0, T, args.saves, out_np, args.cpu_seconds, threads)
+        line["cpu_baseline"] = {
+            "value": cps,
+            "unit": "orbit-steps/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{done} orbits drawn at random from the same workload, {el:.1f} s on {threads} threads",
+        }
+        line["parity_sample_max_rel_err"] = worst
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

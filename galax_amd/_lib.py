@@ -1,0 +1,103 @@
+"""Loader for libgalaxhip.so and the device-memory plumbing around it.
+
+PyTorch-ROCm supplies device memory and the current HIP stream; every
+computation runs in the HIP library.  There is no CPU fallback: if the
+library or a GPU is missing, calls raise.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+from . import _abi
+
+LIB_PATH = Path(__file__).resolve().parent / "libgalaxhip.so"
+_lib = None
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+
+
+class GalaxHipError(RuntimeError):
+    """Error returned by the HIP library."""
+
+
+def _declare(lib: ctypes.CDLL) -> None:
+    sig = {
+        "gx_potential_create": (ctypes.c_int, [_vp, _i32, _vp, _i32, ctypes.POINTER(_vp)]),
+        "gx_potential_destroy": (ctypes.c_int, [_vp]),
+        "gx_acceleration": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _vp]),
+        "gx_gradient": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _vp]),
+        "gx_potential_eval": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _vp]),
+        "gx_hessian": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _vp]),
+        "gx_integrate_workspace_bytes": (_i64, [_i64]),
+        "gx_integrate": (
+            ctypes.c_int,
+            [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+        ),
+        "gx_set_launch_shape": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+        "gx_device_info": (ctypes.c_int, [ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    }
+    optional = {
+        "gx_adjoint_workspace_bytes": (_i64, [_i64, _i32, _i32]),
+        "gx_adjoint": (
+            ctypes.c_int,
+            [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp],
+        ),
+        "gx_mlp_forward": (ctypes.c_int, [_vp, _vp, _i64, _vp, _vp, _vp]),
+        "gx_mlp_train_step": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+        "gx_mlp_describe": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    for name, (res, args) in optional.items():
+        if hasattr(lib, name):
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise GalaxHipError(
+                f"{LIB_PATH} is missing: build it with `python -m galax_amd._build` "
+                "(or __graft_entry__.build())"
+            )
+        _lib = ctypes.CDLL(str(LIB_PATH))
+        _declare(_lib)
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != _abi.GX_OK:
+        names = {-1: "invalid argument", -2: "HIP launch/runtime error", -3: "out of device memory"}
+        raise GalaxHipError(f"{what} failed: {names.get(rc, rc)} ({rc})")
+
+
+def torch():
+    import torch as _torch
+
+    return _torch
+
+
+def require_gpu():
+    t = torch()
+    if not t.cuda.is_available():
+        raise GalaxHipError("galax_amd computes on a ROCm GPU; none is visible to this process")
+    return t
+
+
+def stream_ptr() -> int:
+    t = torch()
+    return int(t.cuda.current_stream().cuda_stream)
+
+
+def ptr(x) -> int | None:
+    return None if x is None else int(x.data_ptr())

@@ -1,0 +1,663 @@
+// gx_integrate.hip -- the orbit integrator of libgalaxhip.so.
+//
+// Replaces the diffrax.diffeqsolve loop that OrbitSolver.solve runs for
+// gd.evaluate_orbit (src/galax/dynamics/_src/orbit/solver.py:774-1039,
+// legacy/funcs.py:42-213), with the batched-t0/t1 semantics: every orbit has
+// its own step sequence.
+//
+// Design (one orbit per lane, persistent launch):
+//   * the grid is sized to the number of resident waves; each lane starts on
+//     orbit `lane`, and when it finishes it takes the next ticket from a
+//     device-wide atomic counter (the work queue), so lanes never idle while
+//     orbits of uneven length remain;
+//   * the state, the stage accelerations (Runge-Kutta-Nystrom form, 3
+//     doubles per stage), the error norm and the PID controller live in
+//     registers; all arithmetic is FP64;
+//   * each loop trip is one evaluation of the method from a base point:
+//     either a controller step from the current state, or a step from the
+//     last accepted step start to the next interior save time; lanes of a
+//     wave therefore run the same instruction stream whatever they do;
+//   * integration backwards in time runs in tau = -t with the velocity sign
+//     flipped, as diffrax does.
+// The controller follows diffrax 0.7.0's PIDController (error_order = method
+// order, RMS norm, factormin applied to rejected steps only), its automatic
+// initial step and its clip-to-end rule.
+#include <cmath>
+
+#include "gx_internal.hpp"
+#include "gx_tableau.hpp"
+
+namespace gx {
+namespace {
+
+struct IArgs {
+  double dt0, rtol, atol, safety, factormin, factormax, dtmin, dtmax;
+  double c1, c2, c3;  // PID exponents (i+p+d)/k, -(p+2d)/k, d/k
+  long long max_steps;
+  int force_dtmin;
+  int t_stride;
+  int T;
+  int ts_mode;
+  int root8;  // c1 == 1/8 and c2 == c3 == 0: factor from three square roots
+  const double* w0;
+  const double* t0;
+  const double* t1;
+  const double* ts;
+  long long n;
+  const long long* order;
+  double* out;
+  int* status;
+  long long* nsteps;
+  long long* naccept;
+  unsigned long long* counter;
+  double* h0;  // first step per orbit (workspace)
+  long long nlanes;
+};
+
+constexpr int kBlock = 256;
+constexpr long long kStepCap = 10000000;  // attempted steps per orbit
+
+// One evaluation of method M from (tb, q, p) with step h (solver time tau;
+// physical time dir * tau, velocities pre-multiplied by dir).
+//
+// The stage loop is kept rolled so that the (large) potential evaluation is
+// emitted once; each stage's coefficients and its slot in g[][] are selected
+// by a chain of compile-time-indexed branches on the (wave-uniform) stage
+// number, which keeps g[][] in registers.
+template <class M, bool FULL, bool ERR>
+__device__ __forceinline__ void rk_eval(const DPot& P, double dir, double tb,
+                                        double h, const double (&q)[3],
+                                        const double (&p)[3], double (&q1)[3],
+                                        double (&p1)[3], double (&eq)[3],
+                                        double (&ep)[3]) {
+  constexpr int S = M::S;
+  double g[S][3];
+  if (!FULL) {
+    // Small evaluator: unroll the stages; every index below is static.
+    accel<FULL>(P, q[0], q[1], q[2], dir * tb, g[0][0], g[0][1], g[0][2]);
+#pragma unroll
+    for (int i = 1; i < S; ++i) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+      for (int l = 0; l < i; ++l) {
+        const double a = M::k.aa[i][l];
+        if (a != 0.0) {
+          s0 = fma(a, g[l][0], s0);
+          s1 = fma(a, g[l][1], s1);
+          s2 = fma(a, g[l][2], s2);
+        }
+      }
+      const double rs = M::k.rs[i];
+      accel<FULL>(P, fma(h, fma(rs, p[0], h * s0), q[0]),
+                  fma(h, fma(rs, p[1], h * s1), q[1]),
+                  fma(h, fma(rs, p[2], h * s2), q[2]),
+                  dir * fma(M::k.c[i], h, tb), g[i][0], g[i][1], g[i][2]);
+    }
+  } else {
+    // Large evaluator: keep the stage loop rolled so that it is emitted
+    // once; the stage's coefficients and its slot in g[][] are selected by
+    // real scalar branches on the wave-uniform stage number.
+#pragma unroll
+    for (int k = 0; k < S; ++k) g[k][0] = g[k][1] = g[k][2] = 0.0;
+    double Qx = q[0], Qy = q[1], Qz = q[2], ts = tb;
+#pragma nounroll
+    for (int i = 0; i < S; ++i) {
+      if (i > 0) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, rs = 0.0, c = 0.0;
+#pragma unroll
+        for (int k = 1; k < S; ++k) {
+          if (k == i) {
+            asm volatile("");  // no if-conversion into selects
+#pragma unroll
+            for (int l = 0; l < k; ++l) {
+              const double a = M::k.aa[k][l];
+              if (a != 0.0) {
+                s0 = fma(a, g[l][0], s0);
+                s1 = fma(a, g[l][1], s1);
+                s2 = fma(a, g[l][2], s2);
+              }
+            }
+            rs = M::k.rs[k];
+            c = M::k.c[k];
+          }
+        }
+        Qx = fma(h, fma(rs, p[0], h * s0), q[0]);
+        Qy = fma(h, fma(rs, p[1], h * s1), q[1]);
+        Qz = fma(h, fma(rs, p[2], h * s2), q[2]);
+        ts = fma(c, h, tb);
+      }
+      double ax, ay, az;
+      accel<FULL>(P, Qx, Qy, Qz, dir * ts, ax, ay, az);
+#pragma unroll
+      for (int k = 0; k < S; ++k) {
+        if (k == i) {
+          asm volatile("");
+          g[k][0] = ax;
+          g[k][1] = ay;
+          g[k][2] = az;
+        }
+      }
+    }
+  }
+  double b0 = 0.0, b1 = 0.0, b2 = 0.0, ba0 = 0.0, ba1 = 0.0, ba2 = 0.0;
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0, ea0 = 0.0, ea1 = 0.0, ea2 = 0.0;
+#pragma unroll
+  for (int l = 0; l < S; ++l) {
+    const double b = M::k.b[l], ba = M::k.ba[l];
+    if (b != 0.0) {
+      b0 = fma(b, g[l][0], b0);
+      b1 = fma(b, g[l][1], b1);
+      b2 = fma(b, g[l][2], b2);
+    }
+    if (ba != 0.0) {
+      ba0 = fma(ba, g[l][0], ba0);
+      ba1 = fma(ba, g[l][1], ba1);
+      ba2 = fma(ba, g[l][2], ba2);
+    }
+    if (ERR) {
+      const double e = M::k.e[l], ea = M::k.ea[l];
+      if (e != 0.0) {
+        e0 = fma(e, g[l][0], e0);
+        e1 = fma(e, g[l][1], e1);
+        e2 = fma(e, g[l][2], e2);
+      }
+      if (ea != 0.0) {
+        ea0 = fma(ea, g[l][0], ea0);
+        ea1 = fma(ea, g[l][1], ea1);
+        ea2 = fma(ea, g[l][2], ea2);
+      }
+    }
+  }
+  constexpr double sb = M::k.sb, se = M::k.se;
+  p1[0] = fma(h, b0, p[0]);
+  p1[1] = fma(h, b1, p[1]);
+  p1[2] = fma(h, b2, p[2]);
+  q1[0] = fma(h, fma(sb, p[0], h * ba0), q[0]);
+  q1[1] = fma(h, fma(sb, p[1], h * ba1), q[1]);
+  q1[2] = fma(h, fma(sb, p[2], h * ba2), q[2]);
+  if (ERR) {
+    ep[0] = h * e0;
+    ep[1] = h * e1;
+    ep[2] = h * e2;
+    eq[0] = h * fma(se, p[0], h * ea0);
+    eq[1] = h * fma(se, p[1], h * ea1);
+    eq[2] = h * fma(se, p[2], h * ea2);
+  }
+}
+
+// diffrax's automatic first step (Hairer, Norsett & Wanner II.4; final
+// exponent 1/(p+1) for a method of order p) for one orbit in solver
+// variables.  Two field evaluations, one call site.
+template <bool FULL>
+__device__ double initial_step(const DPot& P, double dir, double tau0,
+                               const double (&q)[3], const double (&p)[3],
+                               double rtol, double atol, double order) {
+  double sc[6];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    sc[m] = atol + fabs(q[m]) * rtol;
+    sc[m + 3] = atol + fabs(p[m]) * rtol;
+  }
+  double a0[3] = {0.0, 0.0, 0.0}, q1[3], p1[3];
+  double d1 = 0.0, h0 = 0.0, d2 = 0.0;
+#pragma nounroll
+  for (int pass = 0; pass < 2; ++pass) {
+    double x = q[0], y = q[1], z = q[2], t = tau0;
+    if (pass == 1) {
+      x = q1[0];
+      y = q1[1];
+      z = q1[2];
+      t = tau0 + h0;
+    }
+    double ax, ay, az;
+    accel<FULL>(P, x, y, z, dir * t, ax, ay, az);
+    if (pass == 0) {
+      a0[0] = ax;
+      a0[1] = ay;
+      a0[2] = az;
+      double d0 = 0.0;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const double u = q[m] / sc[m], v = p[m] / sc[m + 3];
+        d0 += u * u + v * v;
+        const double fu = p[m] / sc[m], fv = a0[m] / sc[m + 3];
+        d1 += fu * fu + fv * fv;
+      }
+      d0 = sqrt(d0 / 6.0);
+      d1 = sqrt(d1 / 6.0);
+      const bool cond = (d0 < 1e-5) || (d1 < 1e-5);
+      h0 = cond ? 1e-6 : 0.01 * (d0 / (cond ? 1.0 : d1));
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        q1[m] = q[m] + p[m] * h0;
+        p1[m] = p[m] + a0[m] * h0;
+      }
+    } else {
+      const double a1[3] = {ax, ay, az};
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const double u = (p1[m] - p[m]) / sc[m],
+                     v = (a1[m] - a0[m]) / sc[m + 3];
+        d2 += u * u + v * v;
+      }
+      d2 = sqrt(d2 / 6.0) / h0;
+    }
+  }
+  const double maxd = fmax(d1, d2);
+  const double h1 = (maxd <= 1e-15) ? fmax(1e-6, h0 * 1e-3)
+                                    : pow(0.01 / maxd, 1.0 / (order + 1.0));
+  return fmin(100.0 * h0, h1);
+}
+
+__device__ __forceinline__ double clip_to_end(double tprev, double tnext,
+                                              double t1, bool keep) {
+  return (tnext > t1 - 1e-10) ? (keep ? t1 : tprev + 0.5 * (t1 - tprev))
+                              : tnext;
+}
+
+__device__ __forceinline__ double save_time(const IArgs& A, long long orb,
+                                            int s, double t0, double t1) {
+  switch (A.ts_mode) {
+    case GX_TS_SHARED:
+      return A.ts[s];
+    case GX_TS_PER_ORBIT:
+      return A.ts[orb * A.T + s];
+    case GX_TS_LINSPACE:
+      return (s == A.T - 1) ? t1 : t0 + s * ((t1 - t0) / (A.T - 1));
+    default:
+      return t1;
+  }
+}
+
+__device__ __forceinline__ void write_save(const IArgs& A, long long orb,
+                                           int s, double dir,
+                                           const double (&q)[3],
+                                           const double (&p)[3]) {
+  double* o = A.out + (orb * A.T + s) * 6;
+  o[0] = q[0];
+  o[1] = q[1];
+  o[2] = q[2];
+  o[3] = dir * p[0];
+  o[4] = dir * p[1];
+  o[5] = dir * p[2];
+}
+
+__device__ __forceinline__ long long ticket_to_orbit(const IArgs& A,
+                                                     long long idx) {
+  if (idx >= A.n) return -1;
+  if (!A.order) return idx;
+  const long long o = A.order[idx];
+  return (o >= 0 && o < A.n) ? o : -1;  // the host validates permutations
+}
+
+// First-step pre-pass: one lane per orbit computes the step the controller
+// starts with (dt0, or diffrax's automatic choice clipped to [dtmin, dtmax])
+// into h0[orbit], so that the persistent kernel carries no extra copy of the
+// field evaluation.
+template <int ORDER, bool PID, bool FULL>
+__global__ void __launch_bounds__(256) k_first_step(DPot P, IArgs A) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+       i < A.n; i += stride) {
+    const double t0v = A.t0[A.t_stride ? i : 0];
+    const double t1v = A.t1[A.t_stride ? i : 0];
+    const double dir = (t0v < t1v) ? 1.0 : -1.0;
+    double h0 = A.dt0;
+    if (!(h0 > 0.0) && t0v != t1v) {
+      const double* w = A.w0 + i * 6;
+      const double q[3] = {w[0], w[1], w[2]};
+      const double p[3] = {dir * w[3], dir * w[4], dir * w[5]};
+      h0 = initial_step<FULL>(P, dir, dir * t0v, q, p, A.rtol, A.atol,
+                              (double)ORDER);
+    }
+    if (PID) {
+      h0 = fmin(h0, A.dtmax);
+      if (A.dtmin > 0.0) h0 = fmax(h0, A.dtmin);
+    }
+    A.h0[i] = h0;
+  }
+}
+
+// Per-lane state that the step itself does not touch lives in LDS slots
+// (structure of arrays, one column per lane: conflict-free), which keeps the
+// register file for the stage accelerations.
+struct Cold {
+  double qo[6][kBlock];   // state at the start of the last accepted step
+  double tauo[kBlock];    // its time
+  double tau1[kBlock];    // end time (solver time)
+  double t0[kBlock], t1[kBlock];
+  double tsave[kBlock];   // pending interior save time (solver time)
+  double prev_inv[kBlock], prev_prev_inv[kBlock];
+  long long nst[kBlock], nacc[kBlock];
+  int si[kBlock], st[kBlock], at_dtmin[kBlock];
+};
+
+template <class M, bool PID, bool FULL>
+__global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
+  __shared__ Cold C;
+  const int L = threadIdx.x;
+  const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long orb = ticket_to_orbit(A, lane);
+  bool fresh = true;
+
+  double q[3], p[3];
+  double tau = 0.0, tnext = 0.0, dir = 1.0;
+  bool save_mode = false;
+
+  while (orb >= 0) {
+    bool done = false;
+    if (fresh) {
+      fresh = false;
+      const double t0v = A.t0[A.t_stride ? orb : 0];
+      const double t1v = A.t1[A.t_stride ? orb : 0];
+      const double* w = A.w0 + orb * 6;
+      dir = (t0v < t1v) ? 1.0 : -1.0;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        q[m] = w[m];
+        p[m] = dir * w[m + 3];
+        C.qo[m][L] = q[m];
+        C.qo[m + 3][L] = p[m];
+      }
+      tau = dir * t0v;
+      C.tauo[L] = tau;
+      C.tau1[L] = dir * t1v;
+      C.t0[L] = t0v;
+      C.t1[L] = t1v;
+      C.prev_inv[L] = 1.0;
+      C.prev_prev_inv[L] = 1.0;
+      C.nst[L] = 0;
+      C.nacc[L] = 0;
+      C.st[L] = GX_SUCCESS;
+      save_mode = false;
+      int si = 0;
+      if (t0v == t1v) {
+        for (; si < A.T; ++si) write_save(A, orb, si, dir, q, p);
+        done = true;
+      } else {
+        const double h0 = A.h0[orb];
+        C.at_dtmin[L] = (PID && A.dtmin > 0.0 && h0 == A.dtmin) ? 1 : 0;
+        tnext = fmin(tau + h0, dir * t1v);
+        // saves at the start time
+        while (si < A.T && dir * save_time(A, orb, si, t0v, t1v) <= tau) {
+          write_save(A, orb, si, dir, q, p);
+          ++si;
+        }
+      }
+      C.si[L] = si;
+    }
+    if (!done) {
+      if (!save_mode && C.nst[L] >= A.max_steps) {
+        C.st[L] = GX_MAX_STEPS_REACHED;
+        done = true;
+      } else if (!save_mode && !(tnext - tau >= 0.0)) {
+        C.st[L] = GX_NONFINITE;  // NaN step size: stop rather than spin
+        done = true;
+      } else {
+        // ---- one evaluation of the method
+        double tb, h, qb[3], pb[3];
+        if (save_mode) {
+          tb = C.tauo[L];
+          h = C.tsave[L] - tb;
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            qb[m] = C.qo[m][L];
+            pb[m] = C.qo[m + 3][L];
+          }
+        } else {
+          tb = tau;
+          h = tnext - tau;
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            qb[m] = q[m];
+            pb[m] = p[m];
+          }
+        }
+        double q1[3], p1[3], eq[3], ep[3];
+        rk_eval<M, FULL, PID>(P, dir, tb, h, qb, pb, q1, p1, eq, ep);
+
+        const double tau1 = C.tau1[L];
+        if (save_mode) {
+          const int si = C.si[L];
+          write_save(A, orb, si, dir, q1, p1);
+          C.si[L] = si + 1;
+        } else {
+          C.nst[L] += 1;
+          bool finite = true;
+#pragma unroll
+          for (int m = 0; m < 3; ++m)
+            finite = finite && isfinite(q1[m]) && isfinite(p1[m]);
+          if (!finite) {
+            C.st[L] = GX_NONFINITE;
+            done = true;
+          } else {
+            bool keep = true;
+            double next_t1;
+            if (PID) {
+              double sq = 0.0;
+#pragma unroll
+              for (int m = 0; m < 3; ++m) {
+                const double uq =
+                    eq[m] / (A.atol + fmax(fabs(q[m]), fabs(q1[m])) * A.rtol);
+                const double up =
+                    ep[m] / (A.atol + fmax(fabs(p[m]), fabs(p1[m])) * A.rtol);
+                sq += uq * uq + up * up;
+              }
+              const double scaled = sqrt(sq / 6.0);
+              keep = scaled < 1.0;
+              const bool at_dtmin = C.at_dtmin[L] != 0;
+              if (A.dtmin > 0.0) keep = keep || at_dtmin;
+              const double inv = 1.0 / scaled;
+              double f;
+              if (A.root8) {
+                f = sqrt(sqrt(sqrt(inv)));
+              } else {
+                f = (A.c1 == 0.0) ? 1.0 : pow(inv, A.c1);
+                if (A.c2 != 0.0) f *= pow(C.prev_inv[L], A.c2);
+                if (A.c3 != 0.0) f *= pow(C.prev_prev_inv[L], A.c3);
+              }
+              const double fmin_ = keep ? 1.0 : A.factormin;
+              const double factor =
+                  fmin(fmax(A.safety * f, fmin_), A.factormax);
+              double dt = fmin(h * factor, A.dtmax);
+              if (A.dtmin > 0.0) {
+                if (!A.force_dtmin && dt < A.dtmin)
+                  C.st[L] = GX_DT_MIN_REACHED;
+                C.at_dtmin[L] = dt <= A.dtmin ? 1 : 0;
+                dt = fmax(dt, A.dtmin);
+              }
+              if (keep) {
+                // a zero or infinite error does not steer later steps
+                C.prev_prev_inv[L] = C.prev_inv[L];
+                C.prev_inv[L] = (inv == 0.0 || isinf(inv)) ? 1.0 : inv;
+              }
+              next_t1 = (keep ? tnext : tau) + dt;
+            } else {
+              next_t1 = tnext + A.dt0;
+            }
+            if (keep) {
+              C.tauo[L] = tau;
+#pragma unroll
+              for (int m = 0; m < 3; ++m) {
+                C.qo[m][L] = q[m];
+                C.qo[m + 3][L] = p[m];
+                q[m] = q1[m];
+                p[m] = p1[m];
+              }
+              tau = tnext;
+              C.nacc[L] += 1;
+            }
+            const double tprev = fmin(tau, tau1);
+            tnext = clip_to_end(tprev, next_t1, tau1, keep);
+            tau = tprev;
+            if (C.st[L] == GX_DT_MIN_REACHED) done = true;
+          }
+        }
+        if (!done) {
+          // resolve pending saves up to the current state
+          save_mode = false;
+          int si = C.si[L];
+          const double tauo = C.tauo[L];
+          const double t0v = C.t0[L], t1v = C.t1[L];
+          while (si < A.T) {
+            const double ts_tau = dir * save_time(A, orb, si, t0v, t1v);
+            if (ts_tau > tau) break;
+            if (ts_tau == tau) {
+              write_save(A, orb, si, dir, q, p);
+              ++si;
+              continue;
+            }
+            if (ts_tau <= tauo) {
+              const double qo[3] = {C.qo[0][L], C.qo[1][L], C.qo[2][L]};
+              const double po[3] = {C.qo[3][L], C.qo[4][L], C.qo[5][L]};
+              write_save(A, orb, si, dir, qo, po);
+              ++si;
+              continue;
+            }
+            save_mode = true;
+            C.tsave[L] = ts_tau;
+            break;
+          }
+          C.si[L] = si;
+          done = !save_mode && tau >= tau1;
+        }
+      }
+    }
+    if (done) {
+      int si = C.si[L];
+      const int st = C.st[L];
+      if (st != GX_SUCCESS || si < A.T) {
+        const double nanv = __longlong_as_double(0x7ff8000000000000ULL);
+        const double nq[3] = {nanv, nanv, nanv};
+        for (; si < A.T; ++si) write_save(A, orb, si, 1.0, nq, nq);
+      }
+      if (A.status) A.status[orb] = st;
+      if (A.nsteps) A.nsteps[orb] = C.nst[L];
+      if (A.naccept) A.naccept[orb] = C.nacc[L];
+      const long long idx =
+          A.nlanes + (long long)atomicAdd(A.counter, 1ULL);
+      orb = ticket_to_orbit(A, idx);
+      fresh = true;
+    }
+  }
+}
+
+__global__ void k_zero_counter(unsigned long long* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ULL;
+}
+
+template <class M, bool PID, bool FULL>
+int launch(const gx_pot* pot, IArgs& A, hipStream_t stream) {
+  const auto kern = k_integrate<M, PID, FULL>;
+  const LaunchShape& ls = launch_shape();
+  const int block = kBlock;
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, 0) !=
+          hipSuccess ||
+      occ < 1)
+    occ = 1;
+  int per_cu = ls.blocks_per_cu;
+  if (per_cu > occ) per_cu = occ;
+  long long blocks = (long long)num_cus(pot->device) * per_cu;
+  const long long need = (A.n + block - 1) / block;
+  if (blocks > need) blocks = need;
+  if (blocks < 1) blocks = 1;
+  A.nlanes = blocks * block;
+  long long fblocks = (A.n + 255) / 256;
+  const long long fcap = (long long)num_cus(pot->device) * 8;
+  if (fblocks > fcap) fblocks = fcap;
+  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, stream, A.counter);
+  hipLaunchKernelGGL((k_first_step<M::order, PID, FULL>), dim3((unsigned)fblocks),
+                     dim3(256), 0, stream, pot->dev, A);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), 0, stream,
+                     pot->dev, A);
+  return check(hipGetLastError());
+}
+
+}  // namespace
+}  // namespace gx
+
+using namespace gx;
+
+extern "C" {
+
+int64_t gx_integrate_workspace_bytes(int64_t n) {
+  return n < 0 ? 0 : 256 + 8 * n;
+}
+
+int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
+                 const double* w0, const double* t0, const double* t1,
+                 int32_t t_stride, const double* ts, int32_t T,
+                 int32_t ts_mode, int64_t n, const int64_t* order,
+                 double* out, int32_t* status, int64_t* nsteps,
+                 int64_t* naccept, void* workspace, void* stream) {
+  if (!pot || !cfg || n < 0 || T < 1) return GX_EINVAL;
+  if (n == 0) return GX_OK;
+  if (!w0 || !t0 || !t1 || !out || !workspace) return GX_EINVAL;
+  if (t_stride != 0 && t_stride != 1) return GX_EINVAL;
+  if (ts_mode < GX_TS_SHARED || ts_mode > GX_TS_FINAL) return GX_EINVAL;
+  if ((ts_mode == GX_TS_SHARED || ts_mode == GX_TS_PER_ORBIT) && !ts)
+    return GX_EINVAL;
+  if (ts_mode == GX_TS_FINAL && T != 1) return GX_EINVAL;
+  if (cfg->method != GX_DOPRI8 && cfg->method != GX_DOPRI5) return GX_EINVAL;
+  if (cfg->controller != GX_CONSTANT && cfg->controller != GX_PID)
+    return GX_EINVAL;
+  if (cfg->controller == GX_CONSTANT && !(cfg->dt0 > 0.0)) return GX_EINVAL;
+  if (cfg->controller == GX_PID &&
+      !(cfg->rtol >= 0.0 && cfg->atol >= 0.0 && cfg->rtol + cfg->atol > 0.0 &&
+        cfg->safety > 0.0 && cfg->factormin > 0.0 && cfg->factormax >= 1.0))
+    return GX_EINVAL;
+
+  IArgs A{};
+  A.dt0 = cfg->dt0;
+  A.rtol = cfg->rtol;
+  A.atol = cfg->atol;
+  A.safety = cfg->safety;
+  A.factormin = cfg->factormin;
+  A.factormax = cfg->factormax;
+  A.dtmin = cfg->dtmin > 0.0 ? cfg->dtmin : 0.0;
+  A.dtmax = cfg->dtmax > 0.0 ? cfg->dtmax : INFINITY;
+  const double korder = cfg->method == GX_DOPRI8 ? 8.0 : 5.0;
+  A.c1 = (cfg->icoeff + cfg->pcoeff + cfg->dcoeff) / korder;
+  A.c2 = -(cfg->pcoeff + 2.0 * cfg->dcoeff) / korder;
+  A.c3 = cfg->dcoeff / korder;
+  A.root8 = (A.c1 == 0.125 && A.c2 == 0.0 && A.c3 == 0.0) ? 1 : 0;
+  // "unbounded" still ends: every lane stops after kStepCap attempts
+  A.max_steps = (cfg->max_steps > 0 && cfg->max_steps < kStepCap)
+                    ? cfg->max_steps
+                    : kStepCap;
+  A.force_dtmin = cfg->force_dtmin;
+  A.t_stride = t_stride;
+  A.T = T;
+  A.ts_mode = ts_mode;
+  A.w0 = w0;
+  A.t0 = t0;
+  A.t1 = t1;
+  A.ts = ts;
+  A.n = n;
+  A.order = reinterpret_cast<const long long*>(order);
+  A.out = out;
+  A.status = status;
+  A.nsteps = reinterpret_cast<long long*>(nsteps);
+  A.naccept = reinterpret_cast<long long*>(naccept);
+  A.counter = static_cast<unsigned long long*>(workspace);
+  A.h0 = reinterpret_cast<double*>(static_cast<char*>(workspace) + 256);
+
+  hipStream_t s = (hipStream_t)stream;
+  const bool pid = cfg->controller == GX_PID;
+  const bool full = !pot->basic;
+  if (cfg->method == GX_DOPRI8) {
+    if (pid)
+      return full ? launch<Dopri8, true, true>(pot, A, s)
+                  : launch<Dopri8, true, false>(pot, A, s);
+    return full ? launch<Dopri8, false, true>(pot, A, s)
+                : launch<Dopri8, false, false>(pot, A, s);
+  }
+  if (pid)
+    return full ? launch<Dopri5, true, true>(pot, A, s)
+                : launch<Dopri5, true, false>(pot, A, s);
+  return full ? launch<Dopri5, false, true>(pot, A, s)
+              : launch<Dopri5, false, false>(pot, A, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,32 @@
+// gx_internal.hpp -- host-side definitions shared by the library's sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/galax_amd.h"
+#include "gx_potential.hpp"
+
+struct gx_pot {
+  int device = 0;
+  gx::DPot dev{};                 // pointers into `block`
+  void* block = nullptr;          // one device allocation: comps, tracks, knots
+  int ncomp = 0;
+  bool has_track = false;
+  bool has_bar = false;
+  bool basic = true;  // only origin-centred Kepler/Hernquist/MN/NFW/Plummer
+};
+
+namespace gx {
+
+inline int check(hipError_t e) { return e == hipSuccess ? GX_OK : GX_ELAUNCH; }
+
+// Launch-shape defaults, overridable through gx_set_launch_shape.
+struct LaunchShape {
+  int block = 256;
+  int blocks_per_cu = 4;
+};
+LaunchShape& launch_shape();
+
+int num_cus(int device);
+
+}  // namespace gx

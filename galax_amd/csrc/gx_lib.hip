@@ -1,0 +1,363 @@
+// gx_lib.hip -- potential handles and field evaluation kernels of
+// libgalaxhip.so (declared in include/galax_amd.h).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "gx_internal.hpp"
+
+namespace gx {
+
+LaunchShape& launch_shape() {
+  static LaunchShape s;
+  return s;
+}
+
+int num_cus(int device) {
+  static int cached[64] = {0};
+  if (device >= 0 && device < 64 && cached[device] > 0) return cached[device];
+  int n = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount,
+                            device) != hipSuccess ||
+      n <= 0)
+    n = 256;
+  if (device >= 0 && device < 64) cached[device] = n;
+  return n;
+}
+
+namespace {
+
+constexpr int kFieldBlock = 256;
+
+bool finite_all(const double* v, int n) {
+  for (int i = 0; i < n; ++i)
+    if (!std::isfinite(v[i])) return false;
+  return true;
+}
+
+// Derived device constants for one component (see DComp in gx_potential.hpp).
+int derive(const gx_component& in, DComp& out) {
+  std::memset(&out, 0, sizeof(out));
+  out.kind = in.kind;
+  out.frame = in.frame;
+  out.track = in.track;
+  if (!finite_all(in.p, 8) || !finite_all(in.f, 6)) return GX_EINVAL;
+  const double* p = in.p;
+  switch (in.kind) {
+    case GX_NULL:
+      break;
+    case GX_KEPLER:
+      out.p[0] = p[0];
+      break;
+    case GX_HERNQUIST:
+      if (!(p[1] > 0)) return GX_EINVAL;
+      out.p[0] = p[0];
+      out.p[1] = p[1];
+      break;
+    case GX_MIYAMOTO_NAGAI:
+      out.p[0] = p[0];
+      out.p[1] = p[1];
+      out.p[2] = p[2] * p[2];
+      break;
+    case GX_NFW:
+      if (!(p[1] > 0)) return GX_EINVAL;
+      out.p[0] = p[0];
+      out.p[1] = p[1];
+      out.p[2] = 1.0 / p[1];
+      break;
+    case GX_PLUMMER:
+      out.p[0] = p[0];
+      out.p[1] = p[1] * p[1];
+      break;
+    case GX_MONARI_BAR: {
+      const double alpha = p[0], R0 = p[1], v0 = p[2], Rb = p[3], phib = p[4],
+                   Om = p[5];
+      if (!(Rb > 0)) return GX_EINVAL;
+      const double q = R0 / Rb;
+      out.p[0] = alpha * (v0 * v0 / 3.0) * (q * q * q);
+      out.p[1] = Rb;
+      out.p[2] = 1.0 / Rb;
+      out.p[3] = Rb * Rb * Rb;
+      out.p[4] = 1.0 / (Rb * Rb * Rb);
+      out.p[5] = 2.0 * phib;
+      out.p[6] = Om;
+      out.f[4] = std::cos(2.0 * phib);
+      out.f[5] = std::sin(2.0 * phib);
+      break;
+    }
+    default:
+      return GX_EINVAL;
+  }
+  switch (in.frame) {
+    case GX_FRAME_NONE:
+      break;
+    case GX_FRAME_OFFSET:
+    case GX_FRAME_ROTATING:
+      for (int k = 0; k < 4; ++k) out.f[k] = in.f[k];
+      break;
+    case GX_FRAME_TRACK:
+      break;
+    default:
+      return GX_EINVAL;
+  }
+  return GX_OK;
+}
+
+// Rotation rate that lets a component join a co-rotating group, or NaN.
+double group_rate(const gx_component& c) {
+  if (c.frame == GX_FRAME_ROTATING) return c.f[3];
+  if (c.kind == GX_MONARI_BAR && c.frame == GX_FRAME_NONE) return c.p[5];
+  return NAN;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+__global__ void k_accel(DPot P, const double* __restrict__ xyz,
+                        const double* __restrict__ t, int t_stride,
+                        long long n, double* __restrict__ out, double sign) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += stride) {
+    const double ti = t[t_stride ? i : 0];
+    double ax, ay, az;
+    accel<true>(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], ti, ax, ay,
+                 az);
+    out[3 * i] = sign * ax;
+    out[3 * i + 1] = sign * ay;
+    out[3 * i + 2] = sign * az;
+  }
+}
+
+__global__ void k_potential(DPot P, const double* __restrict__ xyz,
+                            const double* __restrict__ t, int t_stride,
+                            long long n, double* __restrict__ out) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += stride) {
+    const double ti = t[t_stride ? i : 0];
+    out[i] = potential(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], ti);
+  }
+}
+
+__global__ void k_hessian(DPot P, const double* __restrict__ xyz,
+                          const double* __restrict__ t, int t_stride,
+                          long long n, double* __restrict__ out) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += stride) {
+    const double ti = t[t_stride ? i : 0];
+    double H[9];
+    hessian(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], ti, H);
+    for (int k = 0; k < 9; ++k) out[9 * i + k] = H[k];
+  }
+}
+
+int field_grid(const gx_pot* pot, long long n) {
+  long long blocks = (n + kFieldBlock - 1) / kFieldBlock;
+  long long cap = (long long)num_cus(pot->device) * 16;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  return (int)blocks;
+}
+
+int field_args_ok(const gx_pot* pot, const void* xyz, const void* t,
+                  int t_stride, long long n, const void* out) {
+  if (!pot || n < 0) return GX_EINVAL;
+  if (n == 0) return GX_OK;
+  if (!xyz || !t || !out || (t_stride != 0 && t_stride != 1)) return GX_EINVAL;
+  return 1;
+}
+
+}  // namespace
+}  // namespace gx
+
+using namespace gx;
+
+extern "C" {
+
+int gx_potential_create(const gx_component* comps, int32_t ncomp,
+                        const gx_track* tracks, int32_t ntracks,
+                        gx_pot** out) {
+  if (!out || ncomp < 0 || ncomp > GX_MAX_COMPONENTS || ntracks < 0 ||
+      ntracks > GX_MAX_TRACKS || (ncomp > 0 && !comps) ||
+      (ntracks > 0 && !tracks))
+    return GX_EINVAL;
+  *out = nullptr;
+  std::vector<DComp> dc(ncomp > 0 ? ncomp : 1);
+  bool has_track = false, has_bar = false;
+  for (int i = 0; i < ncomp; ++i) {
+    if (comps[i].reserved != 0) return GX_EINVAL;
+    if (derive(comps[i], dc[i]) != GX_OK) return GX_EINVAL;
+    if (comps[i].frame == GX_FRAME_TRACK) {
+      if (comps[i].track < 0 || comps[i].track >= ntracks) return GX_EINVAL;
+      has_track = true;
+    }
+    if (comps[i].kind == GX_MONARI_BAR) has_bar = true;
+  }
+  // co-rotating groups: runs of consecutive components with the same rate
+  for (int i = 0; i < ncomp;) {
+    const double w = group_rate(comps[i]);
+    if (std::isnan(w)) {
+      ++i;
+      continue;
+    }
+    int j = i + 1;
+    while (j < ncomp && group_rate(comps[j]) == w) ++j;
+    for (int k = i; k < j; ++k) {
+      dc[k].flags = kInGroup;
+      dc[k].f[3] = w;
+    }
+    dc[i].flags |= kGroupStart;
+    dc[j - 1].flags |= kGroupEnd;
+    i = j;
+  }
+  // tracks: validate knots
+  size_t knot_bytes = 0;
+  for (int k = 0; k < ntracks; ++k) {
+    const gx_track& tr = tracks[k];
+    if (tr.n < 1 || !tr.t || !tr.q || !tr.v) return GX_EINVAL;
+    for (int64_t i = 1; i < tr.n; ++i)
+      if (!(tr.t[i] > tr.t[i - 1])) return GX_EINVAL;
+    for (int64_t i = 0; i < tr.n; ++i)
+      if (!std::isfinite(tr.t[i]) || !std::isfinite(tr.q[3 * i]) ||
+          !std::isfinite(tr.q[3 * i + 1]) || !std::isfinite(tr.q[3 * i + 2]) ||
+          !std::isfinite(tr.v[3 * i]) || !std::isfinite(tr.v[3 * i + 1]) ||
+          !std::isfinite(tr.v[3 * i + 2]))
+        return GX_EINVAL;
+    knot_bytes += align256(sizeof(double) * tr.n) +
+                  2 * align256(sizeof(double) * 3 * tr.n);
+  }
+  const size_t comp_bytes = align256(sizeof(DComp) * dc.size());
+  const size_t track_bytes = align256(sizeof(DTrack) * (ntracks > 0 ? ntracks : 1));
+  const size_t total = comp_bytes + track_bytes + knot_bytes;
+
+  gx_pot* h = new gx_pot();
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return GX_ELAUNCH;
+  }
+  if (hipMalloc(&h->block, total) != hipSuccess) {
+    delete h;
+    return GX_ENOMEM;
+  }
+  char* base = static_cast<char*>(h->block);
+  std::vector<char> host(total, 0);
+  std::memcpy(host.data(), dc.data(), sizeof(DComp) * dc.size());
+  std::vector<DTrack> dt(ntracks > 0 ? ntracks : 1);
+  size_t off = comp_bytes + track_bytes;
+  for (int k = 0; k < ntracks; ++k) {
+    const gx_track& tr = tracks[k];
+    DTrack d{};
+    d.n = tr.n;
+    d.t0 = tr.t[0];
+    d.inv_h = tr.n > 1 ? (double)(tr.n - 1) / (tr.t[tr.n - 1] - tr.t[0]) : 0.0;
+    std::memcpy(host.data() + off, tr.t, sizeof(double) * tr.n);
+    d.t = reinterpret_cast<const double*>(base + off);
+    off += align256(sizeof(double) * tr.n);
+    std::memcpy(host.data() + off, tr.q, sizeof(double) * 3 * tr.n);
+    d.q = reinterpret_cast<const double*>(base + off);
+    off += align256(sizeof(double) * 3 * tr.n);
+    std::memcpy(host.data() + off, tr.v, sizeof(double) * 3 * tr.n);
+    d.v = reinterpret_cast<const double*>(base + off);
+    off += align256(sizeof(double) * 3 * tr.n);
+    dt[k] = d;
+  }
+  std::memcpy(host.data() + comp_bytes, dt.data(), sizeof(DTrack) * dt.size());
+  if (hipMemcpy(h->block, host.data(), total, hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    (void)hipFree(h->block);
+    delete h;
+    return GX_ELAUNCH;
+  }
+  h->dev.comps = reinterpret_cast<const DComp*>(base);
+  h->dev.tracks = reinterpret_cast<const DTrack*>(base + comp_bytes);
+  h->dev.n = ncomp;
+  h->dev.ntracks = ntracks;
+  h->ncomp = ncomp;
+  h->has_track = has_track;
+  h->has_bar = has_bar;
+  h->basic = true;
+  for (int i = 0; i < ncomp; ++i)
+    if (comps[i].frame != GX_FRAME_NONE || comps[i].kind == GX_MONARI_BAR)
+      h->basic = false;
+  *out = h;
+  return GX_OK;
+}
+
+int gx_potential_destroy(gx_pot* pot) {
+  if (!pot) return GX_EINVAL;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(pot->device);
+  (void)hipFree(pot->block);
+  (void)hipSetDevice(prev);
+  delete pot;
+  return GX_OK;
+}
+
+static int launch_accel(const gx_pot* pot, const double* xyz, const double* t,
+                        int32_t t_stride, int64_t n, double* out, void* stream,
+                        double sign) {
+  int ok = field_args_ok(pot, xyz, t, t_stride, n, out);
+  if (ok != 1) return ok;
+  hipLaunchKernelGGL(k_accel, dim3(field_grid(pot, n)), dim3(kFieldBlock), 0,
+                     (hipStream_t)stream, pot->dev, xyz, t, t_stride,
+                     (long long)n, out, sign);
+  return check(hipGetLastError());
+}
+
+int gx_acceleration(const gx_pot* pot, const double* xyz, const double* t,
+                    int32_t t_stride, int64_t n, double* acc, void* stream) {
+  return launch_accel(pot, xyz, t, t_stride, n, acc, stream, 1.0);
+}
+
+int gx_gradient(const gx_pot* pot, const double* xyz, const double* t,
+                int32_t t_stride, int64_t n, double* grad, void* stream) {
+  return launch_accel(pot, xyz, t, t_stride, n, grad, stream, -1.0);
+}
+
+int gx_potential_eval(const gx_pot* pot, const double* xyz, const double* t,
+                      int32_t t_stride, int64_t n, double* phi, void* stream) {
+  int ok = field_args_ok(pot, xyz, t, t_stride, n, phi);
+  if (ok != 1) return ok;
+  hipLaunchKernelGGL(k_potential, dim3(field_grid(pot, n)), dim3(kFieldBlock),
+                     0, (hipStream_t)stream, pot->dev, xyz, t, t_stride,
+                     (long long)n, phi);
+  return check(hipGetLastError());
+}
+
+int gx_hessian(const gx_pot* pot, const double* xyz, const double* t,
+               int32_t t_stride, int64_t n, double* hess, void* stream) {
+  int ok = field_args_ok(pot, xyz, t, t_stride, n, hess);
+  if (ok != 1) return ok;
+  hipLaunchKernelGGL(k_hessian, dim3(field_grid(pot, n)), dim3(kFieldBlock), 0,
+                     (hipStream_t)stream, pot->dev, xyz, t, t_stride,
+                     (long long)n, hess);
+  return check(hipGetLastError());
+}
+
+int gx_set_launch_shape(int32_t block, int32_t blocks_per_cu,
+                        int32_t* prev_block, int32_t* prev_blocks_per_cu) {
+  LaunchShape& s = launch_shape();
+  if (prev_block) *prev_block = s.block;
+  if (prev_blocks_per_cu) *prev_blocks_per_cu = s.blocks_per_cu;
+  if (block < 0 || block > 1024 || (block % 64) != 0) return GX_EINVAL;
+  if (blocks_per_cu < 0 || blocks_per_cu > 64) return GX_EINVAL;
+  if (block > 0) s.block = block;
+  if (blocks_per_cu > 0) s.blocks_per_cu = blocks_per_cu;
+  return GX_OK;
+}
+
+int gx_device_info(int32_t* num_cu, int32_t* warp_size) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return GX_ELAUNCH;
+  if (num_cu) *num_cu = num_cus(dev);
+  if (warp_size) {
+    int w = 64;
+    (void)hipDeviceGetAttribute(&w, hipDeviceAttributeWarpSize, dev);
+    *warp_size = w;
+  }
+  return GX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,445 @@
+// gx_potential.hpp -- device-side evaluation of composite potentials.
+//
+// Closed-form gradients and Hessians of the galax potential models
+// (src/galax/potential/_src/builtin/: kepler.py:94, hernquist.py:80-82,
+// miyamotonagai.py:74-78, nfw/base.py:327-338, plummer.py:123-132,
+// monari2016.py:110-138), summed over components as CompositePotential does
+// (base_multi.py:39-82), with TranslatedPotential's Phi(x - Delta(t))
+// (xfm/translate.py:100-110).  The reference obtains gradients by autodiff of
+// Phi; the forms below are the same derivatives written out.
+//
+// Layout: the component table lives in device memory and every lane of a
+// launch reads the same entries, so the loads are uniform (scalar cache).
+//
+// Rotating groups: consecutive components that rotate rigidly about z at the
+// same rate w (GX_FRAME_ROTATING centres and untranslated Monari bars, whose
+// pattern angle is phi_b + Omega t) are evaluated in the co-rotating frame:
+// the point is rotated once by -w t, each member uses its fixed co-rotating
+// position or bar angle, and the summed acceleration is rotated back.  One
+// sincos per evaluation serves the whole group.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/galax_amd.h"
+
+namespace gx {
+
+constexpr double kTiny = 2.2250738585072014e-308;  // float64 min normal
+
+enum : int { kGroupStart = 1, kGroupEnd = 2, kInGroup = 4 };
+
+// Device component with derived constants:
+//   KEPLER:    p0 = GM
+//   HERNQUIST: p0 = GM, p1 = r_s
+//   MN:        p0 = GM, p1 = a, p2 = b^2
+//   NFW:       p0 = GM, p1 = r_s, p2 = 1/r_s
+//   PLUMMER:   p0 = GM, p1 = r_s^2
+//   MONARI:    p0 = alpha v0^2/3 (R0/Rb)^3, p1 = Rb, p2 = 1/Rb, p3 = Rb^3,
+//              p4 = 1/Rb^3, p5 = 2 phi_b, p6 = Omega,
+//              p7 unused; f4 = cos(2 phi_b), f5 = sin(2 phi_b)
+// Frame block f: OFFSET/ROTATING centre (f0, f1, f2); ROTATING/bar group
+// rate f3.
+struct DComp {
+  int kind;
+  int frame;
+  int track;
+  int flags;  // kGroupStart | kGroupEnd | kInGroup
+  double p[8];
+  double f[6];
+};
+
+struct DTrack {
+  const double* t;
+  const double* q;
+  const double* v;
+  long long n;
+  double t0;
+  double inv_h;  // 1/(mean knot spacing); used to guess the interval
+};
+
+struct DPot {
+  const DComp* comps;
+  const DTrack* tracks;
+  int n;
+  int ntracks;
+};
+
+// ------------------------------------------------------------- tracks
+__device__ inline void track_pos(const DTrack& T, double t, double& x,
+                                 double& y, double& z) {
+  const long long n = T.n;
+  long long k;
+  if (n <= 1 || t <= T.t[0]) {
+    x = T.q[0];
+    y = T.q[1];
+    z = T.q[2];
+    return;
+  }
+  if (t >= T.t[n - 1]) {
+    x = T.q[3 * (n - 1)];
+    y = T.q[3 * (n - 1) + 1];
+    z = T.q[3 * (n - 1) + 2];
+    return;
+  }
+  // guess from the mean spacing, then correct to the knot interval that
+  // holds t (t_k <= t < t_{k+1}).
+  double u = (t - T.t0) * T.inv_h;
+  k = (long long)floor(u);
+  k = k < 0 ? 0 : (k > n - 2 ? n - 2 : k);
+  while (k > 0 && T.t[k] > t) --k;
+  while (k < n - 2 && T.t[k + 1] <= t) ++k;
+  const double tk = T.t[k], hk = T.t[k + 1] - tk;
+  const double s = (t - tk) / hk;
+  const double s2 = s * s, s3 = s2 * s;
+  const double h00 = 2 * s3 - 3 * s2 + 1, h10 = s3 - 2 * s2 + s;
+  const double h01 = -2 * s3 + 3 * s2, h11 = s3 - s2;
+  const double* q0 = T.q + 3 * k;
+  const double* v0 = T.v + 3 * k;
+  x = h00 * q0[0] + h10 * hk * v0[0] + h01 * q0[3] + h11 * hk * v0[3];
+  y = h00 * q0[1] + h10 * hk * v0[1] + h01 * q0[4] + h11 * hk * v0[4];
+  z = h00 * q0[2] + h10 * hk * v0[2] + h01 * q0[5] + h11 * hk * v0[5];
+}
+
+__device__ inline void frame_delta(const DPot& P, const DComp& c, double t,
+                                   double& dx, double& dy, double& dz) {
+  dx = dy = dz = 0.0;
+  if (c.frame == GX_FRAME_OFFSET) {
+    dx = c.f[0];
+    dy = c.f[1];
+    dz = c.f[2];
+  } else if (c.frame == GX_FRAME_ROTATING) {
+    double s, co;
+    sincos(c.f[3] * t, &s, &co);
+    dx = c.f[0] * co - c.f[1] * s;
+    dy = c.f[0] * s + c.f[1] * co;
+    dz = c.f[2];
+  } else if (c.frame == GX_FRAME_TRACK) {
+    track_pos(P.tracks[c.track], t, dx, dy, dz);
+  }
+}
+
+// --------------------------------------------------------- spherical g
+// For a spherical model Phi(r): a = -(Phi'(r)/r) x.  Returns Phi'(r)/r.
+__device__ __forceinline__ double sph_g(int kind, const double* p, double r2,
+                                        double r, double ir) {
+  switch (kind) {
+    case GX_KEPLER:
+      return p[0] * ir * ir * ir;
+    case GX_HERNQUIST: {
+      const double u = 1.0 / (r + p[1]);
+      return p[0] * u * u * ir;
+    }
+    case GX_NFW: {
+      const double L = log1p(r * p[2]);
+      const double u = 1.0 / (r + p[1]);
+      return p[0] * ir * ir * (L * ir - u);
+    }
+    case GX_PLUMMER: {
+      const double id = rsqrt(r2 + p[1]);
+      return p[0] * id * id * id;
+    }
+    default:
+      return 0.0;
+  }
+}
+
+__device__ __forceinline__ void mn_acc(const double* p, double x, double y,
+                                       double z, double& ax, double& ay,
+                                       double& az) {
+  const double zb = z * z + p[2];
+  const double iS = rsqrt(zb);
+  const double S = zb * iS;
+  const double A = p[1] + S;
+  const double iD = rsqrt(x * x + y * y + A * A);
+  const double g = p[0] * iD * iD * iD;
+  ax -= g * x;
+  ay -= g * y;
+  az -= g * z * A * iS;
+}
+
+// Monari bar in a frame where its pattern angle is fixed: c2 = cos 2beta,
+// s2 = sin 2beta.  Phi = P F(r) W, W = (x^2 - y^2) c2 + 2 x y s2,
+// F = U(r/Rb)/r^2 (U(s) = s^3 - 2 for s < 1, -s^-3 otherwise).
+__device__ __forceinline__ void bar_acc(const double* p, double c2, double s2,
+                                        double x, double y, double z,
+                                        double& ax, double& ay, double& az) {
+  const double R2 = x * x + y * y;
+  const double r2 = R2 + z * z;
+  const double ir = rsqrt(r2);
+  const double r = r2 * ir;
+  const double ir2 = ir * ir;
+  const double W = (x * x - y * y) * c2 + 2.0 * x * y * s2;
+  double F, G;  // G = F'(r)/r
+  if (r * p[2] < 1.0) {
+    F = r * p[4] - 2.0 * ir2;
+    G = p[4] * ir + 4.0 * ir2 * ir2;
+  } else {
+    F = -p[3] * ir2 * ir2 * ir;
+    G = 5.0 * p[3] * ir2 * ir2 * ir2 * ir;
+  }
+  const double P = p[0];
+  const double WG = W * G;
+  ax -= P * (F * 2.0 * (x * c2 + y * s2) + WG * x);
+  ay -= P * (F * 2.0 * (x * s2 - y * c2) + WG * y);
+  az -= P * (WG * z);
+}
+
+// Acceleration -grad Phi at (x, y, z, t).  FULL = false compiles only the
+// origin-centred, non-rotating path (the caller guarantees the table has no
+// frames and no bars); FULL = true handles every component.
+template <bool FULL>
+__device__ __forceinline__ void accel(const DPot& P, double x, double y,
+                                      double z, double t, double& ax,
+                                      double& ay, double& az) {
+  ax = ay = az = 0.0;
+  // radius about the origin, shared by untranslated spherical components
+  const double r2o = x * x + y * y + z * z;
+  const double iro = rsqrt(r2o + kTiny);
+  const double ro = (r2o + kTiny) * iro;
+  double gcos = 1.0, gsin = 0.0, xr = x, yr = y;
+  double gax = 0.0, gay = 0.0, gaz = 0.0;
+  for (int i = 0; i < P.n; ++i) {
+    const DComp& c = P.comps[i];
+    const int kind = c.kind;
+    if (!FULL) {
+      if (kind == GX_MIYAMOTO_NAGAI) {
+        mn_acc(c.p, x, y, z, ax, ay, az);
+      } else {
+        const double g = sph_g(kind, c.p, r2o, ro, iro);
+        ax -= g * x;
+        ay -= g * y;
+        az -= g * z;
+      }
+      continue;
+    }
+    const int flags = c.flags;
+    if (flags & kGroupStart) {
+      sincos(c.f[3] * t, &gsin, &gcos);
+      xr = x * gcos + y * gsin;
+      yr = -x * gsin + y * gcos;
+      gax = gay = gaz = 0.0;
+    }
+    if (flags & kInGroup) {
+      // co-rotating frame
+      if (kind == GX_MONARI_BAR) {
+        bar_acc(c.p, c.f[4], c.f[5], xr, yr, z, gax, gay, gaz);
+      } else {
+        const double dx = xr - c.f[0], dy = yr - c.f[1], dz = z - c.f[2];
+        if (kind == GX_MIYAMOTO_NAGAI) {
+          mn_acc(c.p, dx, dy, dz, gax, gay, gaz);
+        } else {
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          const double ir = rsqrt(r2 + kTiny);
+          const double r = (r2 + kTiny) * ir;
+          const double g = sph_g(kind, c.p, r2, r, ir);
+          gax -= g * dx;
+          gay -= g * dy;
+          gaz -= g * dz;
+        }
+      }
+      if (flags & kGroupEnd) {
+        ax += gax * gcos - gay * gsin;
+        ay += gax * gsin + gay * gcos;
+        az += gaz;
+      }
+      continue;
+    }
+    if (c.frame == GX_FRAME_NONE) {
+      if (kind == GX_MIYAMOTO_NAGAI) {
+        mn_acc(c.p, x, y, z, ax, ay, az);
+      } else if (kind == GX_MONARI_BAR) {
+        double s2, c2;
+        sincos(c.p[5] + 2.0 * c.p[6] * t, &s2, &c2);
+        bar_acc(c.p, c2, s2, x, y, z, ax, ay, az);
+      } else if (kind != GX_NULL) {
+        const double g = sph_g(kind, c.p, r2o, ro, iro);
+        ax -= g * x;
+        ay -= g * y;
+        az -= g * z;
+      }
+    } else {
+      double dx, dy, dz;
+      frame_delta(P, c, t, dx, dy, dz);
+      dx = x - dx;
+      dy = y - dy;
+      dz = z - dz;
+      if (kind == GX_MIYAMOTO_NAGAI) {
+        mn_acc(c.p, dx, dy, dz, ax, ay, az);
+      } else if (kind == GX_MONARI_BAR) {
+        double s2, c2;
+        sincos(c.p[5] + 2.0 * c.p[6] * t, &s2, &c2);
+        bar_acc(c.p, c2, s2, dx, dy, dz, ax, ay, az);
+      } else if (kind != GX_NULL) {
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double ir = rsqrt(r2 + kTiny);
+        const double r = (r2 + kTiny) * ir;
+        const double g = sph_g(kind, c.p, r2, r, ir);
+        ax -= g * dx;
+        ay -= g * dy;
+        az -= g * dz;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------ potential and Hessian
+// (not on the integration hot path: each component in the inertial frame)
+__device__ inline double phi_model(const DComp& c, double x, double y,
+                                   double z, double t) {
+  const double* p = c.p;
+  switch (c.kind) {
+    case GX_KEPLER:
+    case GX_HERNQUIST:
+    case GX_NFW:
+    case GX_PLUMMER: {
+      const double r2 = x * x + y * y + z * z;
+      const double r = sqrt(r2 + kTiny);
+      if (c.kind == GX_KEPLER) return -p[0] / r;
+      if (c.kind == GX_HERNQUIST) return -p[0] / (r + p[1]);
+      if (c.kind == GX_NFW) return -p[0] * log1p(r * p[2]) / r;
+      return -p[0] / sqrt(r2 + kTiny + p[1]);
+    }
+    case GX_MIYAMOTO_NAGAI: {
+      const double A = p[1] + sqrt(z * z + p[2]);
+      return -p[0] / sqrt(x * x + y * y + A * A);
+    }
+    case GX_MONARI_BAR: {
+      double s2, c2;
+      sincos(p[5] + 2.0 * p[6] * t, &s2, &c2);
+      const double R2 = x * x + y * y, r2 = R2 + z * z;
+      const double r = sqrt(r2);
+      const double W = (x * x - y * y) * c2 + 2.0 * x * y * s2;
+      const double F = (r * p[2] < 1.0) ? r * p[4] - 2.0 / r2
+                                        : -p[3] / (r2 * r2 * r);
+      return p[0] * F * W;
+    }
+    default:
+      return 0.0;
+  }
+}
+
+// Hessian of one component at (x, y, z) (relative coordinates), added to H
+// (row-major 3x3).
+__device__ inline void hess_model(const DComp& c, double x, double y,
+                                  double z, double t, double* H) {
+  const double* p = c.p;
+  const double X[3] = {x, y, z};
+  switch (c.kind) {
+    case GX_KEPLER:
+    case GX_HERNQUIST:
+    case GX_NFW:
+    case GX_PLUMMER: {
+      // H = g I + (Phi'' - g) xx^T / r^2, g = Phi'/r
+      const double r2 = x * x + y * y + z * z;
+      const double rr = r2 + kTiny;
+      const double r = sqrt(rr);
+      double d1 = 0, d2 = 0;  // Phi', Phi''
+      const double GM = p[0];
+      if (c.kind == GX_KEPLER) {
+        d1 = GM / rr;
+        d2 = -2.0 * GM / (rr * r);
+      } else if (c.kind == GX_HERNQUIST) {
+        const double u = 1.0 / (r + p[1]);
+        d1 = GM * u * u;
+        d2 = -2.0 * GM * u * u * u;
+      } else if (c.kind == GX_NFW) {
+        const double rs = p[1], L = log1p(r * p[2]);
+        const double u = 1.0 / (r + rs);
+        d1 = GM * (L / rr - u / r);
+        d2 = GM * (u / rr - 2.0 * L / (rr * r) + (2.0 * r + rs) * u * u / rr);
+      } else {
+        const double d = rr + p[1];
+        const double id = 1.0 / sqrt(d);
+        d1 = GM * r * id * id * id;
+        d2 = GM * (p[1] - 2.0 * rr) * id * id * id * id * id;
+      }
+      const double g = d1 / r;
+      const double k = (d2 - g) / rr;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          H[3 * a + b] += (a == b ? g : 0.0) + k * X[a] * X[b];
+      break;
+    }
+    case GX_MIYAMOTO_NAGAI: {
+      const double GM = p[0], b2 = p[2];
+      const double S2 = z * z + b2, S = sqrt(S2);
+      const double A = p[1] + S;
+      const double D2 = x * x + y * y + A * A;
+      const double D = sqrt(D2);
+      const double iD3 = 1.0 / (D2 * D), iD5 = iD3 / D2;
+      const double Az = A * z / S;  // d(A^2/2)/dz
+      H[0] += GM * (iD3 - 3.0 * x * x * iD5);
+      H[4] += GM * (iD3 - 3.0 * y * y * iD5);
+      const double hxy = -3.0 * GM * x * y * iD5;
+      H[1] += hxy;
+      H[3] += hxy;
+      const double hxz = -3.0 * GM * x * Az * iD5;
+      const double hyz = -3.0 * GM * y * Az * iD5;
+      H[2] += hxz;
+      H[6] += hxz;
+      H[5] += hyz;
+      H[7] += hyz;
+      H[8] += GM * (z * z * iD3 / S2 + A * b2 * iD3 / (S2 * S) -
+                    3.0 * Az * Az * iD5);
+      break;
+    }
+    case GX_MONARI_BAR: {
+      double s2, c2;
+      sincos(p[5] + 2.0 * p[6] * t, &s2, &c2);
+      const double R2 = x * x + y * y, r2 = R2 + z * z;
+      const double r = sqrt(r2);
+      const double W = (x * x - y * y) * c2 + 2.0 * x * y * s2;
+      const double gW[3] = {2.0 * (x * c2 + y * s2), 2.0 * (x * s2 - y * c2),
+                            0.0};
+      const double HW[9] = {2.0 * c2, 2.0 * s2, 0, 2.0 * s2, -2.0 * c2,
+                            0,        0,        0, 0};
+      double F, F1, F2;
+      if (r * p[2] < 1.0) {
+        F = r * p[4] - 2.0 / r2;
+        F1 = p[4] + 4.0 / (r2 * r);
+        F2 = -12.0 / (r2 * r2);
+      } else {
+        F = -p[3] / (r2 * r2 * r);
+        F1 = 5.0 * p[3] / (r2 * r2 * r2);
+        F2 = -30.0 * p[3] / (r2 * r2 * r2 * r);
+      }
+      const double G = F1 / r;
+      const double k = (F2 - G) / r2;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          const double HF = (a == b ? G : 0.0) + k * X[a] * X[b];
+          H[3 * a + b] += p[0] * (F * HW[3 * a + b] + G * X[a] * gW[b] +
+                                  gW[a] * G * X[b] + W * HF);
+        }
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+__device__ inline double potential(const DPot& P, double x, double y,
+                                   double z, double t) {
+  double s = 0.0;
+  for (int i = 0; i < P.n; ++i) {
+    const DComp& c = P.comps[i];
+    double dx, dy, dz;
+    frame_delta(P, c, t, dx, dy, dz);
+    s += phi_model(c, x - dx, y - dy, z - dz, t);
+  }
+  return s;
+}
+
+__device__ inline void hessian(const DPot& P, double x, double y, double z,
+                               double t, double* H) {
+  for (int k = 0; k < 9; ++k) H[k] = 0.0;
+  for (int i = 0; i < P.n; ++i) {
+    const DComp& c = P.comps[i];
+    double dx, dy, dz;
+    frame_delta(P, c, t, dx, dy, dz);
+    hess_model(c, x - dx, y - dy, z - dz, t, H);
+  }
+}
+
+}  // namespace gx

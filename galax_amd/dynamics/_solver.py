@@ -1,0 +1,231 @@
+"""Solver configuration objects and the integration core.
+
+Mirrors the configuration surface galax passes to diffrax:
+  OrbitSolver(solver, stepsize_controller, max_steps)
+      dynamics/_src/orbit/solver.py:121-141
+  Integrator(dynamics_solver, diffeq_kw)   legacy/integrator.py:43-245
+  Dopri8 / Dopri5, PIDController, ConstantStepSize   (diffrax 0.7.0 names
+  and defaults: PIDController(pcoeff=0, icoeff=1, dcoeff=0, dtmin=None,
+  dtmax=None, force_dtmin=True, factormin=0.2, factormax=10, safety=0.9))
+and runs the HIP integrator (gx_integrate) on device buffers.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from .. import _abi, _lib
+
+__all__ = [
+    "Dopri8",
+    "Dopri5",
+    "PIDController",
+    "ConstantStepSize",
+    "OrbitSolver",
+    "integrate_orbits",
+    "IntegrationResult",
+]
+
+
+@dataclass(frozen=True)
+class Dopri8:
+    """Prince & Dormand RK8(7)13M (diffrax Dopri8)."""
+
+    scan_kind: str | None = None
+    method = _abi.GX_DOPRI8
+    order = 8
+
+
+@dataclass(frozen=True)
+class Dopri5:
+    """Dormand & Prince RK5(4)7M (diffrax Dopri5)."""
+
+    scan_kind: str | None = None
+    method = _abi.GX_DOPRI5
+    order = 5
+
+
+@dataclass(frozen=True)
+class PIDController:
+    rtol: float
+    atol: float
+    pcoeff: float = 0.0
+    icoeff: float = 1.0
+    dcoeff: float = 0.0
+    dtmin: float | None = None
+    dtmax: float | None = None
+    force_dtmin: bool = True
+    factormin: float = 0.2
+    factormax: float = 10.0
+    safety: float = 0.9
+
+
+@dataclass(frozen=True)
+class ConstantStepSize:
+    pass
+
+
+@dataclass(frozen=True)
+class OrbitSolver:
+    """Solver settings; `max_steps=None` means unbounded."""
+
+    solver: object = field(default_factory=Dopri8)
+    stepsize_controller: object = field(default_factory=lambda: PIDController(rtol=1e-8, atol=1e-8))
+    max_steps: int | None = 2**16
+
+    @classmethod
+    def from_(cls, obj):
+        if isinstance(obj, OrbitSolver):
+            return obj
+        if isinstance(obj, dict):
+            return cls(**obj)
+        raise TypeError(f"cannot build an OrbitSolver from {type(obj).__name__}")
+
+    def cfg(self, *, dt0: float | None = None, max_steps="default") -> _abi.GxSolverCfg:
+        c = _abi.GxSolverCfg()
+        c.method = self.solver.method
+        ms = self.max_steps if max_steps == "default" else max_steps
+        c.max_steps = 0 if ms is None else int(ms)
+        ctl = self.stepsize_controller
+        if isinstance(ctl, ConstantStepSize):
+            if dt0 is None or not abs(dt0) > 0:
+                raise ValueError("ConstantStepSize needs a nonzero dt0")
+            c.controller = _abi.GX_CONSTANT
+            c.dt0 = float(abs(dt0))
+            c.rtol = c.atol = 0.0
+            c.safety, c.factormin, c.factormax = 0.9, 0.2, 10.0
+            c.icoeff = 1.0
+        elif isinstance(ctl, PIDController):
+            c.controller = _abi.GX_PID
+            c.dt0 = 0.0 if dt0 is None else float(abs(dt0))
+            c.rtol, c.atol = float(ctl.rtol), float(ctl.atol)
+            c.pcoeff, c.icoeff, c.dcoeff = float(ctl.pcoeff), float(ctl.icoeff), float(ctl.dcoeff)
+            c.safety, c.factormin, c.factormax = float(ctl.safety), float(ctl.factormin), float(ctl.factormax)
+            c.dtmin = 0.0 if ctl.dtmin is None else float(ctl.dtmin)
+            c.dtmax = 0.0 if ctl.dtmax is None else float(ctl.dtmax)
+            c.force_dtmin = int(bool(ctl.force_dtmin))
+        else:
+            raise TypeError(f"unsupported stepsize controller {type(ctl).__name__}")
+        return c
+
+    def with_(self, **kw) -> OrbitSolver:
+        return replace(self, **kw)
+
+
+@dataclass
+class IntegrationResult:
+    """Device-side result of one gx_integrate call (torch tensors)."""
+
+    out: object  # (n, T, 6)
+    status: object  # (n,) int32
+    nsteps: object  # (n,) int64 attempted steps
+    naccept: object  # (n,) int64 accepted steps
+
+    @property
+    def q(self):
+        return self.out[..., :3]
+
+    @property
+    def p(self):
+        return self.out[..., 3:]
+
+
+def _dev(torch, x, dtype):
+    if isinstance(x, torch.Tensor):
+        return x.to(device="cuda", dtype=dtype).contiguous()
+    return torch.as_tensor(np.asarray(x), dtype=dtype, device="cuda").contiguous()
+
+
+def integrate_orbits(
+    pot,
+    w0,
+    t0,
+    t1,
+    *,
+    ts=None,
+    T: int | None = None,
+    ts_mode: int | None = None,
+    solver: OrbitSolver | None = None,
+    dt0: float | None = None,
+    max_steps="default",
+    order=None,
+    out=None,
+    workspace=None,
+    stream=None,
+) -> IntegrationResult:
+    """Integrate n independent orbits on the GPU.
+
+    w0: (n, 6); t0, t1: scalars or (n,); ts: (T,) shared save times, (n, T)
+    per-orbit save times, or None (with `T`: per-orbit linspace(t0, t1, T);
+    without: the final state only).  Returns device tensors."""
+    torch = _lib.require_gpu()
+    solver = solver or OrbitSolver()
+    cfg = solver.cfg(dt0=dt0, max_steps=max_steps)
+    w0d = _dev(torch, w0, torch.float64)
+    if w0d.ndim != 2 or w0d.shape[1] != 6:
+        raise ValueError(f"w0 must have shape (n, 6), got {tuple(w0d.shape)}")
+    n = int(w0d.shape[0])
+    t0d = _dev(torch, t0, torch.float64).reshape(-1)
+    t1d = _dev(torch, t1, torch.float64).reshape(-1)
+    if t0d.numel() == 1 and t1d.numel() == 1:
+        t_stride = 0
+    else:
+        t0d = torch.broadcast_to(t0d, (n,)).contiguous()
+        t1d = torch.broadcast_to(t1d, (n,)).contiguous()
+        t_stride = 1
+    tsd = None
+    if ts is not None:
+        tsd = _dev(torch, ts, torch.float64)
+        if tsd.ndim == 1:
+            mode, T = _abi.GX_TS_SHARED, int(tsd.shape[0])
+        elif tsd.ndim == 2 and tsd.shape[0] == n:
+            mode, T = _abi.GX_TS_PER_ORBIT, int(tsd.shape[1])
+        else:
+            raise ValueError("ts must have shape (T,) or (n, T)")
+    elif T is not None:
+        mode = _abi.GX_TS_LINSPACE
+    else:
+        mode, T = _abi.GX_TS_FINAL, 1
+    if ts_mode is not None:
+        mode = ts_mode
+    if T < 1:
+        raise ValueError("need at least one save time")
+    if out is None:
+        out = torch.empty((n, T, 6), dtype=torch.float64, device="cuda")
+    status = torch.empty((n,), dtype=torch.int32, device="cuda")
+    nsteps = torch.empty((n,), dtype=torch.int64, device="cuda")
+    naccept = torch.empty((n,), dtype=torch.int64, device="cuda")
+    lib = _lib.lib()
+    if workspace is None:
+        nbytes = int(lib.gx_integrate_workspace_bytes(n))
+        workspace = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device="cuda")
+    orderd = None
+    if order is not None:
+        orderd = _dev(torch, order, torch.int64)
+        if orderd.shape != (n,) or not bool(
+            torch.equal(torch.sort(orderd).values, torch.arange(n, device="cuda", dtype=torch.int64))
+        ):
+            raise ValueError("order must be a permutation of range(n)")
+    rc = lib.gx_integrate(
+        pot._handle(),
+        __import__("ctypes").addressof(cfg),
+        _lib.ptr(w0d),
+        _lib.ptr(t0d),
+        _lib.ptr(t1d),
+        t_stride,
+        _lib.ptr(tsd),
+        T,
+        mode,
+        n,
+        _lib.ptr(orderd),
+        _lib.ptr(out),
+        _lib.ptr(status),
+        _lib.ptr(nsteps),
+        _lib.ptr(naccept),
+        _lib.ptr(workspace),
+        stream if stream is not None else _lib.stream_ptr(),
+    )
+    _lib.check(rc, "gx_integrate")
+    return IntegrationResult(out=out, status=status, nsteps=nsteps, naccept=naccept)

@@ -1,0 +1,181 @@
+"""Analytic potential models on galax's orbit hot path.
+
+Each class keeps the reference's constructor parameters and units and maps
+them to a component-table row (include/galax_amd.h):
+
+  KeplerPotential(m_tot)              potential/_src/builtin/kepler.py:34-94
+  HernquistPotential(m_tot, r_s)      builtin/hernquist.py:27-82
+  MiyamotoNagaiPotential(m_tot, a, b) builtin/miyamotonagai.py:30-78
+  NFWPotential(m, r_s)                builtin/nfw/base.py:36-100,327-338
+  PlummerPotential(m_tot, r_s)        builtin/plummer.py:33-132
+  MonariEtAl2016BarPotential(...)     builtin/monari2016.py:29-138
+  NullPotential()                     builtin/null.py
+
+Parameters are numbers in galactic units (kpc, Myr, Msun, rad) or `Q`
+quantities.  Mass parameters enter the table as G*m.
+"""
+
+from __future__ import annotations
+
+import math
+
+from .. import _abi
+from ..units import G, Q, to_float
+from ._base import AbstractSinglePotential
+
+__all__ = [
+    "KeplerPotential",
+    "HernquistPotential",
+    "MiyamotoNagaiPotential",
+    "NFWPotential",
+    "PlummerPotential",
+    "MonariEtAl2016BarPotential",
+    "NullPotential",
+]
+
+
+def _p(x, unit: str | None = None) -> float:
+    if isinstance(x, Q):
+        return float(x.to_value(unit))
+    return to_float(x)
+
+
+class _MassScale(AbstractSinglePotential):
+    _mass_name = "m_tot"
+
+    def __init__(self, m_tot, r_s, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m_tot = _p(m_tot, "Msun")
+        self.r_s = _p(r_s, "kpc")
+
+    def _params(self):
+        return (G * self.m_tot, self.r_s)
+
+    def __repr__(self):
+        return f"{type(self).__name__}(m_tot={self.m_tot!r}, r_s={self.r_s!r})"
+
+
+class KeplerPotential(AbstractSinglePotential):
+    """Point mass: Phi = -G m_tot / r."""
+
+    kind = _abi.GX_KEPLER
+
+    def __init__(self, m_tot, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m_tot = _p(m_tot, "Msun")
+
+    def _params(self):
+        return (G * self.m_tot,)
+
+    def __repr__(self):
+        return f"KeplerPotential(m_tot={self.m_tot!r})"
+
+
+class HernquistPotential(_MassScale):
+    """Phi = -G m_tot / (r + r_s)."""
+
+    kind = _abi.GX_HERNQUIST
+
+
+class PlummerPotential(_MassScale):
+    """Phi = -G m_tot / sqrt(r^2 + r_s^2)."""
+
+    kind = _abi.GX_PLUMMER
+
+
+class NFWPotential(AbstractSinglePotential):
+    """Phi = -(G m / r_s) log(1 + r/r_s) / (r/r_s)."""
+
+    kind = _abi.GX_NFW
+
+    def __init__(self, m, r_s, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m = _p(m, "Msun")
+        self.r_s = _p(r_s, "kpc")
+
+    def _params(self):
+        return (G * self.m, self.r_s)
+
+    def __repr__(self):
+        return f"NFWPotential(m={self.m!r}, r_s={self.r_s!r})"
+
+
+class MiyamotoNagaiPotential(AbstractSinglePotential):
+    """Phi = -G m_tot / sqrt(R^2 + (a + sqrt(z^2 + b^2))^2)."""
+
+    kind = _abi.GX_MIYAMOTO_NAGAI
+
+    def __init__(self, m_tot, a, b, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m_tot = _p(m_tot, "Msun")
+        self.a = _p(a, "kpc")
+        self.b = _p(b, "kpc")
+
+    def _params(self):
+        return (G * self.m_tot, self.a, self.b)
+
+    def __repr__(self):
+        return f"MiyamotoNagaiPotential(m_tot={self.m_tot!r}, a={self.a!r}, b={self.b!r})"
+
+
+class MonariEtAl2016BarPotential(AbstractSinglePotential):
+    """Rotating bar of Monari et al. (2016):
+    Phi = alpha (v0^2/3) (R0/Rb)^3 U(r/Rb) (R^2/r^2) cos 2(phi - phi_b - Omega t).
+
+    Defaults follow the reference: alpha = 0.01, Rb = 3.5 kpc,
+    phi_b = 25 deg, Omega = 52.2 km/s/kpc; R0 and v0 have no default."""
+
+    kind = _abi.GX_MONARI_BAR
+
+    def __init__(
+        self,
+        *,
+        alpha=0.01,
+        R0,
+        v0,
+        Rb=3.5,
+        phi_b=Q(25.0, "deg"),
+        Omega=Q(52.2, "km/(s kpc)"),
+        units="galactic",
+    ):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.alpha = _p(alpha)
+        self.R0 = _p(R0, "kpc")
+        self.v0 = _p(v0, "kpc/Myr")
+        self.Rb = _p(Rb, "kpc")
+        self.phi_b = _p(phi_b, "rad")
+        self.Omega = _p(Omega, "1/Myr")
+
+    def _params(self):
+        return (self.alpha, self.R0, self.v0, self.Rb, self.phi_b, self.Omega)
+
+    def __repr__(self):
+        return (
+            f"MonariEtAl2016BarPotential(alpha={self.alpha!r}, R0={self.R0!r}, v0={self.v0!r}, "
+            f"Rb={self.Rb!r}, phi_b={self.phi_b!r}, Omega={self.Omega!r})"
+        )
+
+
+class NullPotential(AbstractSinglePotential):
+    """Phi = 0."""
+
+    kind = _abi.GX_NULL
+
+    def __init__(self, *, units="galactic"):
+        pass
+
+    def _params(self):
+        return ()
+
+    def __repr__(self):
+        return "NullPotential()"
+
+
+# angle helper used by callers that build bars from degrees
+def deg(x: float) -> float:
+    return math.radians(x)

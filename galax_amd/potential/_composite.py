@@ -1,0 +1,162 @@
+"""Composite, preset and translated potentials.
+
+  CompositePotential      potential/_src/composite.py:33-111 (sum of
+                          components, base_multi.py:39-82)
+  MilkyWayPotential       builtin/milkyway.py:173-236 (disk, halo, bulge,
+                          nucleus, summed in that order)
+  TranslatedPotential     xfm/translate.py:29-110: Phi(x - Delta(t))
+  TimeDependentTranslationParameter  xfm/translate.py:117-199
+"""
+
+from __future__ import annotations
+
+from collections.abc import Mapping
+
+import numpy as np
+
+from .. import _abi
+from ..units import Q, ustrip
+from ._base import AbstractPotential, Track
+from ._builtin import HernquistPotential, MiyamotoNagaiPotential, NFWPotential
+
+__all__ = [
+    "CompositePotential",
+    "MilkyWayPotential",
+    "TranslatedPotential",
+    "TimeDependentTranslationParameter",
+    "RotatingTranslation",
+]
+
+
+class CompositePotential(AbstractPotential, Mapping):
+    """Ordered sum of named potentials."""
+
+    _preset = False
+
+    def __init__(self, _data: Mapping | None = None, /, **kwargs):
+        data = dict(_data or {})
+        data.update(kwargs)
+        for k, v in data.items():
+            if not isinstance(v, AbstractPotential):
+                raise TypeError(f"component {k!r} is not a potential")
+        self._data = data
+
+    def __getitem__(self, k):
+        return self._data[k]
+
+    def __iter__(self):
+        return iter(self._data)
+
+    def __len__(self):
+        return len(self._data)
+
+    def _rows(self):
+        rows = []
+        for v in self._data.values():
+            rows.extend(v._rows())
+        return rows
+
+    def __repr__(self):
+        inner = ", ".join(f"{k}={v!r}" for k, v in self._data.items())
+        return f"{type(self).__name__}({inner})"
+
+
+def _as(cls, v, defaults: dict):
+    if isinstance(v, cls):
+        return v
+    if v is None:
+        return cls(**defaults)
+    kw = dict(defaults)
+    kw.update(v)
+    return cls(**kw)
+
+
+class MilkyWayPotential(CompositePotential):
+    """Bovy (2015) disk + NFW halo + Hernquist bulge + Hernquist nucleus with
+    galax's default parameters."""
+
+    _preset = True
+    DISK = {"m_tot": 6.8e10, "a": 3.0, "b": 0.28}
+    HALO = {"m": 5.4e11, "r_s": 15.62}
+    BULGE = {"m_tot": 5e9, "r_s": 1.0}
+    NUCLEUS = {"m_tot": 1.71e9, "r_s": 0.07}
+
+    def __init__(self, *, disk=None, halo=None, bulge=None, nucleus=None, units="galactic"):
+        super().__init__(
+            disk=_as(MiyamotoNagaiPotential, disk, self.DISK),
+            halo=_as(NFWPotential, halo, self.HALO),
+            bulge=_as(HernquistPotential, bulge, self.BULGE),
+            nucleus=_as(HernquistPotential, nucleus, self.NUCLEUS),
+        )
+
+
+class RotatingTranslation:
+    """Delta(t) = Rz(Omega t) . xyz0: a centre fixed in a frame that rotates
+    rigidly about z (a circular orbit of radius |xyz0| when z0 = 0)."""
+
+    def __init__(self, xyz0, Omega):
+        self.xyz0 = np.asarray(ustrip(xyz0, "kpc") if isinstance(xyz0, Q) else xyz0, dtype=np.float64)
+        self.Omega = float(ustrip(Omega, "1/Myr") if isinstance(Omega, Q) else Omega)
+
+    def __call__(self, t):
+        t = np.asarray(t, dtype=np.float64)
+        c, s = np.cos(self.Omega * t), np.sin(self.Omega * t)
+        x0, y0, z0 = self.xyz0
+        return np.stack([x0 * c - y0 * s, x0 * s + y0 * c, np.broadcast_to(z0, np.shape(t))], axis=-1)
+
+
+class TimeDependentTranslationParameter:
+    """Tabulated centre track.  The reference fits an interpax cubic spline to
+    positions (xfm/translate.py:185-199); here the track is a cubic Hermite
+    interpolant of positions and velocities at the knots.  Without
+    velocities, they are estimated with second-order finite differences."""
+
+    def __init__(self, t, q, v=None):
+        t = np.asarray(ustrip(t, "Myr") if isinstance(t, Q) else t, dtype=np.float64)
+        q = np.asarray(ustrip(q, "kpc") if isinstance(q, Q) else q, dtype=np.float64)
+        order = np.argsort(t)
+        t, q = t[order], q[order]
+        if v is None:
+            v = np.gradient(q, t, axis=0, edge_order=2)
+        else:
+            v = np.asarray(ustrip(v, "kpc/Myr") if isinstance(v, Q) else v, dtype=np.float64)[order]
+        self.track = Track(t=t, q=q, v=v)
+
+    @classmethod
+    def from_(cls, t, q, v=None):
+        return cls(t, q, v)
+
+
+class TranslatedPotential(AbstractPotential):
+    """Phi_base(x - Delta(t)).  `translation` is a constant offset (3,), a
+    RotatingTranslation, or a TimeDependentTranslationParameter."""
+
+    def __init__(self, original_potential: AbstractPotential, translation, *, units="galactic"):
+        self.original_potential = original_potential
+        self.translation = translation
+
+    def _rows(self):
+        rows = []
+        for row, track in self.original_potential._rows():
+            if row.frame != _abi.GX_FRAME_NONE or track is not None:
+                raise ValueError("nested translations are not supported")
+            tr = self.translation
+            if isinstance(tr, RotatingTranslation):
+                row.frame = _abi.GX_FRAME_ROTATING
+                for i, v in enumerate((*tr.xyz0, tr.Omega)):
+                    row.f[i] = float(v)
+                rows.append((row, None))
+            elif isinstance(tr, TimeDependentTranslationParameter):
+                rows.append((row, tr.track))
+            else:
+                off = np.asarray(ustrip(tr, "kpc") if isinstance(tr, Q) else tr, dtype=np.float64)
+                if off.shape != (3,):
+                    raise ValueError("a constant translation has shape (3,)")
+                row.frame = _abi.GX_FRAME_OFFSET
+                for i in range(3):
+                    row.f[i] = float(off[i])
+                rows.append((row, None))
+        return rows
+
+    def __repr__(self):
+        return f"TranslatedPotential({self.original_potential!r}, {type(self.translation).__name__})"

@@ -1,0 +1,92 @@
+"""Seeded synthetic workloads for the benchmark configurations.
+
+All ICs come from NumPy's PCG64 with a recorded seed and are handed to the
+GPU and to any CPU checker alike.  Circular speeds for the ICs use a small
+NumPy closed form of the Milky Way model's radial force (host-side set-up
+only; the integration itself runs on the GPU).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .units import G
+
+MW_DEFAULT = {
+    "disk": {"m_tot": 6.8e10, "a": 3.0, "b": 0.28},
+    "halo": {"m": 5.4e11, "r_s": 15.62},
+    "bulge": {"m_tot": 5e9, "r_s": 1.0},
+    "nucleus": {"m_tot": 1.71e9, "r_s": 0.07},
+}
+
+
+def mw_gradient_np(xyz: np.ndarray, mw: dict | None = None) -> np.ndarray:
+    """grad Phi of the Milky Way model (MN disk + NFW + 2 Hernquist)."""
+    p = mw or MW_DEFAULT
+    x, y, z = xyz[..., 0], xyz[..., 1], xyz[..., 2]
+    r = np.sqrt(x * x + y * y + z * z)
+    g = np.zeros_like(xyz)
+    d = p["disk"]
+    S = np.sqrt(z * z + d["b"] ** 2)
+    A = d["a"] + S
+    D3 = (x * x + y * y + A * A) ** 1.5
+    gm = G * d["m_tot"]
+    g[..., 0] += gm * x / D3
+    g[..., 1] += gm * y / D3
+    g[..., 2] += gm * z * A / (S * D3)
+    h = p["halo"]
+    gm = G * h["m"]
+    dphidr = gm * (np.log1p(r / h["r_s"]) / r**2 - 1.0 / (r * (r + h["r_s"])))
+    g += (dphidr / r)[..., None] * xyz
+    for k in ("bulge", "nucleus"):
+        gm = G * p[k]["m_tot"]
+        dphidr = gm / (r + p[k]["r_s"]) ** 2
+        g += (dphidr / r)[..., None] * xyz
+    return g
+
+
+def disk_ics(n: int, seed: int, R_range=(4.0, 20.0), z_sigma=0.3, dispersion=0.1, mw=None) -> np.ndarray:
+    """(n, 6) ICs: R ~ U(R_range), phi ~ U(0, 2 pi), z ~ N(0, z_sigma),
+    v = v_c phi_hat + N(0, dispersion * v_c) in each component, with
+    v_c = sqrt(r |dPhi/dr|) (galax local_circular_velocity)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    R = rng.uniform(R_range[0], R_range[1], n)
+    phi = rng.uniform(0.0, 2.0 * np.pi, n)
+    z = rng.normal(0.0, z_sigma, n)
+    xyz = np.stack([R * np.cos(phi), R * np.sin(phi), z], axis=-1)
+    r = np.linalg.norm(xyz, axis=-1)
+    g = mw_gradient_np(xyz, mw)
+    vc = np.sqrt(r * np.abs(np.sum(g * xyz, axis=-1) / r))
+    phat = np.stack([-np.sin(phi), np.cos(phi), np.zeros(n)], axis=-1)
+    v = vc[:, None] * phat + rng.normal(0.0, 1.0, (n, 3)) * (dispersion * vc)[:, None]
+    return np.concatenate([xyz, v], axis=-1)
+
+
+def config_plain_small(seed: int = 1):
+    """1,024 MW orbits, Dopri8 fixed step 0.5 Myr over 0..1000 Myr.  Save
+    times lie on the step grid (every 20 steps: 0, 10, ..., 990 Myr) plus
+    the end point."""
+    w0 = disk_ics(1024, seed)
+    ts = np.concatenate([np.arange(0.0, 1000.0, 10.0), [1000.0]])
+    return w0, 0.0, 1000.0, ts
+
+
+def config_uneven(n: int = 1_048_576, seed: int = 2):
+    """1M MW orbits: R ~ U(0.5, 50) kpc, 25% dispersion, per-orbit
+    integration time log-uniform in 0.1..5 Gyr, 64 save times per orbit
+    (linspace(0, T_i, 64)); adaptive Dopri8 rtol 1e-8, atol 1e-10."""
+    w0 = disk_ics(n, seed, R_range=(0.5, 50.0), dispersion=0.25)
+    rng = np.random.Generator(np.random.PCG64(seed + 1000))
+    T = 10.0 ** rng.uniform(np.log10(100.0), np.log10(5000.0), n)
+    return w0, np.zeros(n), T
+
+
+def cost_order(w0: np.ndarray, T: np.ndarray, mw=None) -> np.ndarray:
+    """Longest-first work-queue order from a cost model: steps scale with
+    the integration time over the local dynamical time r / v_c."""
+    xyz = w0[:, :3]
+    r = np.linalg.norm(xyz, axis=-1)
+    g = mw_gradient_np(xyz, mw)
+    vc = np.sqrt(r * np.abs(np.sum(g * xyz, axis=-1) / np.maximum(r, 1e-12)))
+    cost = T * vc / np.maximum(r, 1e-3)
+    return np.argsort(-cost, kind="stable").astype(np.int64)

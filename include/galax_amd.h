@@ -1,0 +1,212 @@
+/*
+ * galax_amd.h -- C ABI of libgalaxhip.so, the MI355X (gfx950) orbit engine.
+ *
+ * Units everywhere are galax's "galactic" system: kpc, Myr, Msun, rad.
+ * G = 4.498502151469553e-12 kpc^3 Msun^-1 Myr^-2.
+ *
+ * Conventions (all entry points):
+ *   - every pointer argument that names an array is a DEVICE pointer owned by
+ *     the caller (torch.Tensor.data_ptr() in the Python layer), except the
+ *     component/track tables handed to gx_potential_create, which are host
+ *     arrays copied into the handle;
+ *   - `stream` is a hipStream_t (NULL = legacy default stream); every compute
+ *     call is asynchronous on it and returns after the launch;
+ *   - return value 0 = ok, < 0 = argument or launch error (see GX_E*); nothing
+ *     throws across the ABI;
+ *   - handles are immutable after creation: concurrent calls that share a
+ *     handle are safe as long as they use distinct output buffers.
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the galax source tree, src/galax/...).
+ */
+#ifndef GALAX_AMD_H
+#define GALAX_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+#define GX_OK 0
+#define GX_EINVAL (-1)   /* bad argument (null pointer, bad size, bad enum) */
+#define GX_ELAUNCH (-2)  /* HIP launch / runtime error */
+#define GX_ENOMEM (-3)   /* device allocation failed (create calls only) */
+
+/* ------------------------------------------------------------ components */
+/* Potential models (potential/_src/builtin/...).  Parameter block p[]:
+ *   GX_KEPLER          p0 = G*m_tot                         kepler.py:34-94
+ *   GX_HERNQUIST       p0 = G*m_tot, p1 = r_s               hernquist.py:27-82
+ *   GX_MIYAMOTO_NAGAI  p0 = G*m_tot, p1 = a, p2 = b         miyamotonagai.py:30-78
+ *   GX_NFW             p0 = G*m,     p1 = r_s               nfw/base.py:36-100,327-338
+ *   GX_PLUMMER         p0 = G*m_tot, p1 = r_s               plummer.py:33-132
+ *   GX_MONARI_BAR      p0 = alpha, p1 = R0, p2 = v0 [kpc/Myr], p3 = Rb,
+ *                      p4 = phi_b [rad], p5 = Omega [rad/Myr] monari2016.py:29-138
+ *   GX_NULL            no parameters (Phi = 0)              null.py
+ */
+enum gx_kind {
+  GX_NULL = 0,
+  GX_KEPLER = 1,
+  GX_HERNQUIST = 2,
+  GX_MIYAMOTO_NAGAI = 3,
+  GX_NFW = 4,
+  GX_PLUMMER = 5,
+  GX_MONARI_BAR = 6
+};
+
+/* Where the component sits (potential/_src/xfm/translate.py:29-199 —
+ * TranslatedPotential evaluates Phi_base(x - Delta(t))).  Frame block f[]:
+ *   GX_FRAME_NONE      centred on the origin.
+ *   GX_FRAME_OFFSET    Delta = (f0, f1, f2), constant.
+ *   GX_FRAME_ROTATING  Delta(t) = Rz(f3 * t) . (f0, f1, f2): a centre fixed in
+ *                      a frame rotating rigidly about z at f3 rad/Myr (a
+ *                      circular orbit when f2 = 0).
+ *   GX_FRAME_TRACK     Delta(t) = cubic Hermite interpolant of track
+ *                      number `track` (positions and velocities at knots). */
+enum gx_frame {
+  GX_FRAME_NONE = 0,
+  GX_FRAME_OFFSET = 1,
+  GX_FRAME_ROTATING = 2,
+  GX_FRAME_TRACK = 3
+};
+
+typedef struct gx_component {
+  int32_t kind;     /* enum gx_kind */
+  int32_t frame;    /* enum gx_frame */
+  int32_t track;    /* track index when frame == GX_FRAME_TRACK, else 0 */
+  int32_t reserved; /* must be 0 */
+  double p[8];      /* model parameters (see gx_kind) */
+  double f[6];      /* frame parameters (see gx_frame) */
+} gx_component;
+
+/* A tabulated centre trajectory: n knots, strictly increasing t[n],
+ * positions q[n*3] and velocities v[n*3] (row-major).  Outside [t0, t_{n-1}]
+ * the end knot is held (constant position). */
+typedef struct gx_track {
+  int64_t n;
+  const double* t;
+  const double* q;
+  const double* v;
+} gx_track;
+
+typedef struct gx_pot gx_pot; /* opaque handle; lives on one device */
+
+/* Build a device-resident potential from a host component table.
+ * Replaces the construction of a CompositePotential / AbstractPotential
+ * pytree (potential/_src/composite.py:33-111, base_multi.py:39-82): the
+ * composite is the sum of its components, evaluated in table order.
+ * Host arrays; copied.  At most GX_MAX_COMPONENTS components. */
+#define GX_MAX_COMPONENTS 256
+#define GX_MAX_TRACKS 8
+int gx_potential_create(const gx_component* comps, int32_t ncomp,
+                        const gx_track* tracks, int32_t ntracks,
+                        gx_pot** out);
+int gx_potential_destroy(gx_pot* pot);
+
+/* ----------------------------------------------------------- field calls */
+/* Positions xyz[n*3] (row-major), times t[n] when t_stride == 1 or a single
+ * time t[0] when t_stride == 0.  Outputs row-major.
+ *
+ * gx_acceleration  replaces AbstractPotential.acceleration /
+ *                  gp.acceleration (potential/_src/base.py:250-255,
+ *                  register_funcs.py:327-344), i.e. -grad Phi.
+ * gx_gradient      replaces AbstractPotential._gradient (base.py:170-179).
+ * gx_potential_eval replaces AbstractPotential._potential / gp.potential
+ *                  (base.py:113-136).
+ * gx_hessian       replaces AbstractPotential._hessian (base.py:230-239);
+ *                  out[n*9] row-major 3x3. */
+int gx_acceleration(const gx_pot* pot, const double* xyz, const double* t,
+                    int32_t t_stride, int64_t n, double* acc, void* stream);
+int gx_gradient(const gx_pot* pot, const double* xyz, const double* t,
+                int32_t t_stride, int64_t n, double* grad, void* stream);
+int gx_potential_eval(const gx_pot* pot, const double* xyz, const double* t,
+                      int32_t t_stride, int64_t n, double* phi, void* stream);
+int gx_hessian(const gx_pot* pot, const double* xyz, const double* t,
+               int32_t t_stride, int64_t n, double* hess, void* stream);
+
+/* ----------------------------------------------------------- integration */
+enum gx_method { GX_DOPRI8 = 0, GX_DOPRI5 = 1 };
+enum gx_controller { GX_CONSTANT = 0, GX_PID = 1 };
+
+/* Solver configuration: mirrors OrbitSolver(solver, stepsize_controller,
+ * max_steps) (dynamics/_src/orbit/solver.py:121-141) with diffrax's
+ * PIDController / ConstantStepSize fields.  Unset optional bounds use
+ * dtmin = 0, dtmax = +inf.  dt0 <= 0 selects the automatic initial step. */
+typedef struct gx_solver_cfg {
+  int32_t method;      /* enum gx_method */
+  int32_t controller;  /* enum gx_controller */
+  int64_t max_steps;   /* attempted-step cap per orbit; <= 0 = "unbounded",
+                          which is a safety cap of 10,000,000 */
+  double dt0;          /* first step (required for GX_CONSTANT) */
+  double rtol, atol;
+  double pcoeff, icoeff, dcoeff;
+  double safety, factormin, factormax;
+  double dtmin, dtmax;
+  int32_t force_dtmin; /* accept a step taken at dtmin even if it fails */
+  int32_t reserved;
+} gx_solver_cfg;
+
+/* Per-orbit status, mirroring diffrax RESULTS. */
+enum gx_status {
+  GX_SUCCESS = 0,
+  GX_MAX_STEPS_REACHED = 1,
+  GX_DT_MIN_REACHED = 2,
+  GX_NONFINITE = 3
+};
+
+/* How save times are given.
+ *   GX_TS_SHARED    ts[T], the same for every orbit (evaluate_orbit's `t`);
+ *   GX_TS_PER_ORBIT ts[n*T] row-major (experimental integrate_orbit's (B,T)
+ *                   saveat, dynamics/_src/experimental/integrate.py:446-570);
+ *   GX_TS_LINSPACE  ts ignored; orbit i saves at linspace(t0[i], t1[i], T);
+ *   GX_TS_FINAL     ts ignored, T must be 1; save the state at t1[i]. */
+enum gx_ts_mode {
+  GX_TS_SHARED = 0,
+  GX_TS_PER_ORBIT = 1,
+  GX_TS_LINSPACE = 2,
+  GX_TS_FINAL = 3
+};
+
+/* Integrate n independent orbits of Hamilton's equations dq/dt = p,
+ * dp/dt = -grad Phi(q, t) (dynamics/_src/orbit/field_hamiltonian.py:240-249)
+ * from t0[i] to t1[i] (t_stride 0: one shared pair t0[0], t1[0]), starting
+ * from w0[n*6] = (x,y,z,vx,vy,vz), writing out[n*T*6] at the save times.
+ * Each orbit takes its own step sequence (the batched-t0/t1 semantics of
+ * orbit/solver.py:854-1039).  The state at an interior save time is the
+ * method's own solution of a step from the last accepted step start to that
+ * time.  Replaces the diffeqsolve call inside OrbitSolver.solve
+ * (orbit/solver.py:774-1039) and thus the hot loop of gd.evaluate_orbit
+ * (legacy/funcs.py:42-213).
+ *
+ * Optional outputs (may be NULL): status[n] (enum gx_status), nsteps[n]
+ * (attempted steps), naccept[n] (accepted steps).
+ * `order` (optional, may be NULL) is a permutation of 0..n-1 giving the order
+ * in which the persistent work queue hands out orbits; longest-first order
+ * shortens the tail of a launch of uneven orbits.
+ * `workspace` must point to at least gx_integrate_workspace_bytes(n) bytes of
+ * device memory (256-byte aligned) that no concurrent call uses; it holds the
+ * work-queue counter and each orbit's first step. */
+int64_t gx_integrate_workspace_bytes(int64_t n);
+int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
+                 const double* w0, const double* t0, const double* t1,
+                 int32_t t_stride, const double* ts, int32_t T,
+                 int32_t ts_mode, int64_t n, const int64_t* order,
+                 double* out, int32_t* status, int64_t* nsteps,
+                 int64_t* naccept, void* workspace, void* stream);
+
+/* Launch-shape override used by the autotuner: workgroup size (multiple of
+ * 64, <= 1024) and workgroups per CU for the persistent integrator; 0 keeps
+ * the built-in default.  Returns the previous values through the pointers
+ * when they are non-NULL.  Not thread-safe with concurrent launches. */
+int gx_set_launch_shape(int32_t block, int32_t blocks_per_cu,
+                        int32_t* prev_block, int32_t* prev_blocks_per_cu);
+
+/* Library and device information for tooling: number of compute units of
+ * the current device, and the VGPR count of the integrator kernels. */
+int gx_device_info(int32_t* num_cu, int32_t* warp_size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GALAX_AMD_H */

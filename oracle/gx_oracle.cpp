@@ -1,0 +1,919 @@
+// gx_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU FP64 restatement of galax's orbit hot path, used as the checker for the
+// HIP library and as the CPU baseline in bench.py.  Nothing in the shipped
+// package links or calls this file.
+//
+// What it restates (paths relative to galax's src/galax/):
+//   * potentials, written as the reference writes Phi, and differentiated by
+//     forward-mode automatic differentiation (a jet of value, gradient and
+//     Hessian) -- the same route as AbstractPotential._gradient = jax.grad(
+//     _potential) (potential/_src/base.py:170-179) and _hessian =
+//     jax.hessian(_potential) (base.py:230-239):
+//       Kepler          builtin/kepler.py:94
+//       Hernquist       builtin/hernquist.py:80-82
+//       Miyamoto-Nagai  builtin/miyamotonagai.py:74-78
+//       NFW             builtin/nfw/base.py:327-338
+//       Plummer         builtin/plummer.py:123-132
+//       Monari bar      builtin/monari2016.py:110-138
+//     with r = sqrt(x^2+y^2+z^2 + tiny) (potential/_src/utils.py:45-126);
+//   * the composite sum of component gradients (base_multi.py:39-82);
+//   * translation Phi(x - Delta(t)) (xfm/translate.py:100-110);
+//   * Hamilton's equations (dynamics/_src/orbit/field_hamiltonian.py:240-249);
+//   * the explicit Runge-Kutta step with embedded error, the PID step-size
+//     controller, the automatic initial step, the clip-to-end rule and the
+//     constant step size of diffrax 0.7.0 (third-party, not vendored; the
+//     call sites are dynamics/_src/orbit/solver.py:121-141,774-812 and
+//     legacy/integrator.py:161-174).  The published algorithms are restated:
+//     Prince & Dormand (1981) RK8(7)13M for Dopri8, Dormand & Prince (1980)
+//     RK5(4)7M for Dopri5, Hairer-Norsett-Wanner's initial-step rule and the
+//     PID formula of Soderlind as parameterised by diffrax;
+//   * both batch semantics: independent per-orbit solves
+//     (orbit/solver.py:854-1039) and one shared solve over a (B,6) state
+//     (orbit/solver.py:774-812).
+// Interior save times are filled by a step of the method from the last
+// accepted step start to the save time (the GPU library does the same);
+// diffrax's own Dopri8 interpolant is not restated (parity unpinned there).
+
+#include "../include/galax_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr double kTiny = 2.2250738585072014e-308;  // float64 min normal
+
+// ------------------------------------------------------------------- jets
+// Value, gradient and (ORD == 2) Hessian with respect to (x, y, z).
+template <int ORD>
+struct J {
+  double v = 0;
+  double g[3] = {0, 0, 0};
+  double h[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+};
+
+template <int O>
+J<O> jconst(double c) {
+  J<O> r;
+  r.v = c;
+  return r;
+}
+template <int O>
+J<O> jvar(double v, int i) {
+  J<O> r;
+  r.v = v;
+  r.g[i] = 1.0;
+  return r;
+}
+template <int O>
+J<O> operator+(const J<O>& a, const J<O>& b) {
+  J<O> r;
+  r.v = a.v + b.v;
+  for (int i = 0; i < 3; ++i) r.g[i] = a.g[i] + b.g[i];
+  if (O == 2)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) r.h[i][j] = a.h[i][j] + b.h[i][j];
+  return r;
+}
+template <int O>
+J<O> operator-(const J<O>& a) {
+  J<O> r;
+  r.v = -a.v;
+  for (int i = 0; i < 3; ++i) r.g[i] = -a.g[i];
+  if (O == 2)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) r.h[i][j] = -a.h[i][j];
+  return r;
+}
+template <int O>
+J<O> operator-(const J<O>& a, const J<O>& b) {
+  return a + (-b);
+}
+template <int O>
+J<O> operator+(const J<O>& a, double c) {
+  J<O> r = a;
+  r.v = a.v + c;
+  return r;
+}
+template <int O>
+J<O> operator+(double c, const J<O>& a) {
+  return a + c;
+}
+template <int O>
+J<O> operator-(const J<O>& a, double c) {
+  J<O> r = a;
+  r.v = a.v - c;
+  return r;
+}
+template <int O>
+J<O> operator-(double c, const J<O>& a) {
+  return (-a) + c;
+}
+template <int O>
+J<O> operator*(const J<O>& a, double c) {
+  J<O> r;
+  r.v = a.v * c;
+  for (int i = 0; i < 3; ++i) r.g[i] = a.g[i] * c;
+  if (O == 2)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) r.h[i][j] = a.h[i][j] * c;
+  return r;
+}
+template <int O>
+J<O> operator*(double c, const J<O>& a) {
+  return a * c;
+}
+template <int O>
+J<O> operator*(const J<O>& a, const J<O>& b) {
+  J<O> r;
+  r.v = a.v * b.v;
+  for (int i = 0; i < 3; ++i) r.g[i] = a.v * b.g[i] + b.v * a.g[i];
+  if (O == 2)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        r.h[i][j] = a.v * b.h[i][j] + b.v * a.h[i][j] + a.g[i] * b.g[j] +
+                    b.g[i] * a.g[j];
+  return r;
+}
+template <int O>
+J<O> operator/(const J<O>& a, const J<O>& b) {
+  // q = a/b:  q' = (a' - q b')/b,  q'' = (a'' - q b'' - q'b'^T - b'q'^T)/b
+  J<O> r;
+  r.v = a.v / b.v;
+  for (int i = 0; i < 3; ++i) r.g[i] = (a.g[i] - r.v * b.g[i]) / b.v;
+  if (O == 2)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        r.h[i][j] = (a.h[i][j] - r.v * b.h[i][j] - r.g[i] * b.g[j] -
+                     b.g[i] * r.g[j]) /
+                    b.v;
+  return r;
+}
+template <int O>
+J<O> operator/(const J<O>& a, double c) {
+  J<O> r;
+  r.v = a.v / c;
+  for (int i = 0; i < 3; ++i) r.g[i] = a.g[i] / c;
+  if (O == 2)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) r.h[i][j] = a.h[i][j] / c;
+  return r;
+}
+template <int O>
+J<O> operator/(double c, const J<O>& b) {
+  return jconst<O>(c) / b;
+}
+// Chain rule for a scalar function with value f, first and second derivative.
+template <int O>
+J<O> chain(const J<O>& a, double f, double d1, double d2) {
+  J<O> r;
+  r.v = f;
+  for (int i = 0; i < 3; ++i) r.g[i] = d1 * a.g[i];
+  if (O == 2)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        r.h[i][j] = d1 * a.h[i][j] + d2 * a.g[i] * a.g[j];
+  return r;
+}
+template <int O>
+J<O> jsqrt(const J<O>& a) {
+  double s = std::sqrt(a.v);
+  return chain(a, s, 0.5 / s, -0.25 / (s * a.v));
+}
+template <int O>
+J<O> jlog1p(const J<O>& a) {
+  double d = 1.0 / (1.0 + a.v);
+  return chain(a, std::log1p(a.v), d, -d * d);
+}
+template <int O>
+J<O> jcos(const J<O>& a) {
+  double c = std::cos(a.v), s = std::sin(a.v);
+  return chain(a, c, -s, -c);
+}
+template <int O>
+J<O> jatan2(const J<O>& y, const J<O>& x) {
+  // F(y, x) = atan2(y, x); Fy = x/r2, Fx = -y/r2,
+  // Fyy = -2xy/r2^2, Fxx = 2xy/r2^2, Fxy = (y^2 - x^2)/r2^2.
+  double r2 = x.v * x.v + y.v * y.v;
+  double Fy = x.v / r2, Fx = -y.v / r2;
+  double Fyy = -2 * x.v * y.v / (r2 * r2), Fxx = -Fyy;
+  double Fxy = (y.v * y.v - x.v * x.v) / (r2 * r2);
+  J<O> r;
+  r.v = std::atan2(y.v, x.v);
+  for (int i = 0; i < 3; ++i) r.g[i] = Fy * y.g[i] + Fx * x.g[i];
+  if (O == 2)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        r.h[i][j] = Fy * y.h[i][j] + Fx * x.h[i][j] + Fyy * y.g[i] * y.g[j] +
+                    Fxx * x.g[i] * x.g[j] +
+                    Fxy * (y.g[i] * x.g[j] + x.g[i] * y.g[j]);
+  return r;
+}
+template <int O>
+J<O> sq(const J<O>& a) {  // jnp x**2 (integer_pow)
+  return a * a;
+}
+template <int O>
+J<O> cube(const J<O>& a) {  // x**3
+  return a * a * a;
+}
+
+// ------------------------------------------------------- potential models
+template <int O>
+J<O> r_spherical(const J<O>& x, const J<O>& y, const J<O>& z) {
+  return jsqrt(sq(x) + sq(y) + sq(z) + kTiny);
+}
+
+template <int O>
+J<O> phi_model(const gx_component& c, const J<O>& x, const J<O>& y,
+               const J<O>& z, double t) {
+  const double* p = c.p;
+  switch (c.kind) {
+    case GX_KEPLER: {  // -G m / r
+      J<O> r = r_spherical(x, y, z);
+      return (-p[0]) / r;
+    }
+    case GX_HERNQUIST: {  // -G m / (r + r_s)
+      J<O> r = r_spherical(x, y, z);
+      return (-p[0]) / (r + p[1]);
+    }
+    case GX_MIYAMOTO_NAGAI: {  // -G m / sqrt(R^2 + (sqrt(z^2+b^2) + a)^2)
+      J<O> R2 = sq(x) + sq(y);
+      J<O> zp2 = sq(jsqrt(sq(z) + p[2] * p[2]) + p[1]);
+      return (-p[0]) / jsqrt(R2 + zp2);
+    }
+    case GX_NFW: {  // -(G m / r_s) log1p(r/r_s) / (r/r_s)
+      J<O> r = r_spherical(x, y, z);
+      J<O> s = r / p[1];
+      double phi0 = -p[0] / p[1];
+      return phi0 * jlog1p(s) / s;
+    }
+    case GX_PLUMMER: {  // -G m / sqrt(r^2 + r_s^2)
+      J<O> r = r_spherical(x, y, z);
+      return (-p[0]) / jsqrt(sq(r) + p[1] * p[1]);
+    }
+    case GX_MONARI_BAR: {
+      // prefactor * U(r/Rb) * (R^2/r^2) * cos(2 (phi - phi_b - Omega t))
+      double alpha = p[0], R0 = p[1], v0 = p[2], Rb = p[3], phib = p[4],
+             Om = p[5];
+      J<O> R2 = sq(x) + sq(y);
+      J<O> r2 = R2 + sq(z);
+      double q = R0 / Rb;
+      double pref = alpha * (v0 * v0 / 3.0) * (q * q * q);
+      J<O> s = jsqrt(r2) / Rb;
+      J<O> U = (s.v >= 1.0) ? -(1.0 / cube(s)) : cube(s) - 2.0;
+      J<O> ph = jatan2(y, x);
+      J<O> gam = 2.0 * (ph - phib - Om * t);
+      return pref * U * (R2 / r2) * jcos(gam);
+    }
+    default:
+      return jconst<O>(0.0);
+  }
+}
+
+struct Track {
+  int64_t n = 0;
+  const double* t = nullptr;
+  const double* q = nullptr;
+  const double* v = nullptr;
+};
+
+// Cubic Hermite interpolant of a tabulated trajectory (end knots held).
+void track_pos(const Track& tr, double t, double out[3]) {
+  if (tr.n <= 0) {
+    out[0] = out[1] = out[2] = 0;
+    return;
+  }
+  if (tr.n == 1 || t <= tr.t[0]) {
+    for (int k = 0; k < 3; ++k) out[k] = tr.q[k];
+    return;
+  }
+  if (t >= tr.t[tr.n - 1]) {
+    for (int k = 0; k < 3; ++k) out[k] = tr.q[3 * (tr.n - 1) + k];
+    return;
+  }
+  // largest i with t_i <= t
+  int64_t lo = 0, hi = tr.n - 1;
+  while (hi - lo > 1) {
+    int64_t mid = (lo + hi) / 2;
+    if (tr.t[mid] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  double h = tr.t[lo + 1] - tr.t[lo];
+  double u = (t - tr.t[lo]) / h;
+  double u2 = u * u, u3 = u2 * u;
+  double h00 = 2 * u3 - 3 * u2 + 1, h10 = u3 - 2 * u2 + u;
+  double h01 = -2 * u3 + 3 * u2, h11 = u3 - u2;
+  for (int k = 0; k < 3; ++k)
+    out[k] = h00 * tr.q[3 * lo + k] + h10 * h * tr.v[3 * lo + k] +
+             h01 * tr.q[3 * (lo + 1) + k] + h11 * h * tr.v[3 * (lo + 1) + k];
+}
+
+struct Pot {
+  std::vector<gx_component> comps;
+  std::vector<Track> tracks;
+};
+
+void frame_offset(const Pot& P, const gx_component& c, double t,
+                  double d[3]) {
+  d[0] = d[1] = d[2] = 0;
+  switch (c.frame) {
+    case GX_FRAME_OFFSET:
+      d[0] = c.f[0];
+      d[1] = c.f[1];
+      d[2] = c.f[2];
+      break;
+    case GX_FRAME_ROTATING: {
+      double a = c.f[3] * t, ca = std::cos(a), sa = std::sin(a);
+      d[0] = c.f[0] * ca - c.f[1] * sa;
+      d[1] = c.f[0] * sa + c.f[1] * ca;
+      d[2] = c.f[2];
+      break;
+    }
+    case GX_FRAME_TRACK:
+      if (c.track >= 0 && c.track < (int)P.tracks.size())
+        track_pos(P.tracks[c.track], t, d);
+      break;
+    default:
+      break;
+  }
+}
+
+template <int O>
+J<O> phi_component(const Pot& P, const gx_component& c, const double xyz[3],
+                   double t) {
+  double d[3];
+  frame_offset(P, c, t, d);
+  J<O> x = jvar<O>(xyz[0] - d[0], 0);
+  J<O> y = jvar<O>(xyz[1] - d[1], 1);
+  J<O> z = jvar<O>(xyz[2] - d[2], 2);
+  return phi_model<O>(c, x, y, z, t);
+}
+
+double potential(const Pot& P, const double xyz[3], double t) {
+  double s = 0;
+  for (const auto& c : P.comps) s += phi_component<1>(P, c, xyz, t).v;
+  return s;
+}
+
+void gradient(const Pot& P, const double xyz[3], double t, double g[3]) {
+  g[0] = g[1] = g[2] = 0;
+  for (const auto& c : P.comps) {
+    J<1> j = phi_component<1>(P, c, xyz, t);
+    for (int k = 0; k < 3; ++k) g[k] += j.g[k];
+  }
+}
+
+void hessian(const Pot& P, const double xyz[3], double t, double H[9]) {
+  for (int k = 0; k < 9; ++k) H[k] = 0;
+  for (const auto& c : P.comps) {
+    J<2> j = phi_component<2>(P, c, xyz, t);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) H[3 * a + b] += j.h[a][b];
+  }
+}
+
+// ------------------------------------------------------------- tableaux
+struct Tableau {
+  int s = 0;      // stages
+  int order = 0;  // order used by the controller (diffrax error_order)
+  double c[13] = {};
+  double a[13][13] = {};
+  double b[13] = {};
+  double be[13] = {};  // b_sol - b_embedded
+};
+
+Tableau make_dopri8() {
+  // Prince & Dormand (1981), RK8(7)13M.
+  Tableau T;
+  T.s = 13;
+  T.order = 8;
+  const double c[13] = {0.0,
+                        1.0 / 18,
+                        1.0 / 12,
+                        1.0 / 8,
+                        5.0 / 16,
+                        3.0 / 8,
+                        59.0 / 400,
+                        93.0 / 200,
+                        5490023248.0 / 9719169821.0,
+                        13.0 / 20,
+                        1201146811.0 / 1299019798.0,
+                        1.0,
+                        1.0};
+  for (int i = 0; i < 13; ++i) T.c[i] = c[i];
+  auto& a = T.a;
+  a[1][0] = 1.0 / 18;
+  a[2][0] = 1.0 / 48;
+  a[2][1] = 1.0 / 16;
+  a[3][0] = 1.0 / 32;
+  a[3][2] = 3.0 / 32;
+  a[4][0] = 5.0 / 16;
+  a[4][2] = -75.0 / 64;
+  a[4][3] = 75.0 / 64;
+  a[5][0] = 3.0 / 80;
+  a[5][3] = 3.0 / 16;
+  a[5][4] = 3.0 / 20;
+  a[6][0] = 29443841.0 / 614563906.0;
+  a[6][3] = 77736538.0 / 692538347.0;
+  a[6][4] = -28693883.0 / 1125000000.0;
+  a[6][5] = 23124283.0 / 1800000000.0;
+  a[7][0] = 16016141.0 / 946692911.0;
+  a[7][3] = 61564180.0 / 158732637.0;
+  a[7][4] = 22789713.0 / 633445777.0;
+  a[7][5] = 545815736.0 / 2771057229.0;
+  a[7][6] = -180193667.0 / 1043307555.0;
+  a[8][0] = 39632708.0 / 573591083.0;
+  a[8][3] = -433636366.0 / 683701615.0;
+  a[8][4] = -421739975.0 / 2616292301.0;
+  a[8][5] = 100302831.0 / 723423059.0;
+  a[8][6] = 790204164.0 / 839813087.0;
+  a[8][7] = 800635310.0 / 3783071287.0;
+  a[9][0] = 246121993.0 / 1340847787.0;
+  a[9][3] = -37695042795.0 / 15268766246.0;
+  a[9][4] = -309121744.0 / 1061227803.0;
+  a[9][5] = -12992083.0 / 490766935.0;
+  a[9][6] = 6005943493.0 / 2108947869.0;
+  a[9][7] = 393006217.0 / 1396673457.0;
+  a[9][8] = 123872331.0 / 1001029789.0;
+  a[10][0] = -1028468189.0 / 846180014.0;
+  a[10][3] = 8478235783.0 / 508512852.0;
+  a[10][4] = 1311729495.0 / 1432422823.0;
+  a[10][5] = -10304129995.0 / 1701304382.0;
+  a[10][6] = -48777925059.0 / 3047939560.0;
+  a[10][7] = 15336726248.0 / 1032824649.0;
+  a[10][8] = -45442868181.0 / 3398467696.0;
+  a[10][9] = 3065993473.0 / 597172653.0;
+  a[11][0] = 185892177.0 / 718116043.0;
+  a[11][3] = -3185094517.0 / 667107341.0;
+  a[11][4] = -477755414.0 / 1098053517.0;
+  a[11][5] = -703635378.0 / 230739211.0;
+  a[11][6] = 5731566787.0 / 1027545527.0;
+  a[11][7] = 5232866602.0 / 850066563.0;
+  a[11][8] = -4093664535.0 / 808688257.0;
+  a[11][9] = 3962137247.0 / 1805957418.0;
+  a[11][10] = 65686358.0 / 487910083.0;
+  a[12][0] = 403863854.0 / 491063109.0;
+  a[12][3] = -5068492393.0 / 434740067.0;
+  a[12][4] = -411421997.0 / 543043805.0;
+  a[12][5] = 652783627.0 / 914296604.0;
+  a[12][6] = 11173962825.0 / 925320556.0;
+  a[12][7] = -13158990841.0 / 6184727034.0;
+  a[12][8] = 3936647629.0 / 1978049680.0;
+  a[12][9] = -160528059.0 / 685178525.0;
+  a[12][10] = 248638103.0 / 1413531060.0;
+  const double b[13] = {14005451.0 / 335480064.0,
+                        0,
+                        0,
+                        0,
+                        0,
+                        -59238493.0 / 1068277825.0,
+                        181606767.0 / 758867731.0,
+                        561292985.0 / 797845732.0,
+                        -1041891430.0 / 1371343529.0,
+                        760417239.0 / 1151165299.0,
+                        118820643.0 / 751138087.0,
+                        -528747749.0 / 2220607170.0,
+                        1.0 / 4};
+  const double bh[13] = {13451932.0 / 455176623.0,
+                         0,
+                         0,
+                         0,
+                         0,
+                         -808719846.0 / 976000145.0,
+                         1757004468.0 / 5645159321.0,
+                         656045339.0 / 265891186.0,
+                         -3867574721.0 / 1518517206.0,
+                         465885868.0 / 322736535.0,
+                         53011238.0 / 667516719.0,
+                         2.0 / 45,
+                         0};
+  for (int i = 0; i < 13; ++i) {
+    T.b[i] = b[i];
+    T.be[i] = b[i] - bh[i];
+  }
+  return T;
+}
+
+Tableau make_dopri5() {
+  // Dormand & Prince (1980), RK5(4)7M, with the embedded weights diffrax
+  // uses for its error estimate.
+  Tableau T;
+  T.s = 7;
+  T.order = 5;
+  const double c[7] = {0, 1.0 / 5, 3.0 / 10, 4.0 / 5, 8.0 / 9, 1.0, 1.0};
+  for (int i = 0; i < 7; ++i) T.c[i] = c[i];
+  auto& a = T.a;
+  a[1][0] = 1.0 / 5;
+  a[2][0] = 3.0 / 40;
+  a[2][1] = 9.0 / 40;
+  a[3][0] = 44.0 / 45;
+  a[3][1] = -56.0 / 15;
+  a[3][2] = 32.0 / 9;
+  a[4][0] = 19372.0 / 6561;
+  a[4][1] = -25360.0 / 2187;
+  a[4][2] = 64448.0 / 6561;
+  a[4][3] = -212.0 / 729;
+  a[5][0] = 9017.0 / 3168;
+  a[5][1] = -355.0 / 33;
+  a[5][2] = 46732.0 / 5247;
+  a[5][3] = 49.0 / 176;
+  a[5][4] = -5103.0 / 18656;
+  a[6][0] = 35.0 / 384;
+  a[6][2] = 500.0 / 1113;
+  a[6][3] = 125.0 / 192;
+  a[6][4] = -2187.0 / 6784;
+  a[6][5] = 11.0 / 84;
+  const double b[7] = {35.0 / 384, 0, 500.0 / 1113, 125.0 / 192,
+                       -2187.0 / 6784, 11.0 / 84, 0};
+  const double be[7] = {35.0 / 384 - 1951.0 / 21600,
+                        0,
+                        500.0 / 1113 - 22642.0 / 50085,
+                        125.0 / 192 - 451.0 / 720,
+                        -2187.0 / 6784 + 12231.0 / 42400,
+                        11.0 / 84 - 649.0 / 6300,
+                        -1.0 / 60.0};
+  for (int i = 0; i < 7; ++i) {
+    T.b[i] = b[i];
+    T.be[i] = be[i];
+  }
+  return T;
+}
+
+const Tableau& tableau(int method) {
+  static const Tableau d8 = make_dopri8();
+  static const Tableau d5 = make_dopri5();
+  return method == GX_DOPRI5 ? d5 : d8;
+}
+
+// -------------------------------------------------------------- stepping
+// Vector field of Hamilton's equations in the solver's time variable
+// tau = dir * t: dy/dtau = dir * (p, -grad Phi(q, dir*tau)).
+struct Field {
+  const Pot* P;
+  double dir;
+  void operator()(double tau, const double* y, double* f) const {
+    double g[3];
+    gradient(*P, y, dir * tau, g);
+    f[0] = dir * y[3];
+    f[1] = dir * y[4];
+    f[2] = dir * y[5];
+    f[3] = -dir * g[0];
+    f[4] = -dir * g[1];
+    f[5] = -dir * g[2];
+  }
+};
+
+// One explicit RK step of size h from (tau, y[6*B]) for B orbits sharing the
+// step: stage increments k = h * f, Y_i = y + sum_j a_ij k_j,
+// y1 = y + sum_i b_i k_i, err = sum_i be_i k_i.
+void rk_step(const Tableau& T, const Field& F, double tau, const double* y,
+             double h, int64_t B, double* y1, double* err) {
+  const int s = T.s;
+  std::vector<double> k((size_t)s * 6 * B), Y(6 * B), fb(6);
+  for (int i = 0; i < s; ++i) {
+    double ti = tau + T.c[i] * h;
+    for (int64_t o = 0; o < B; ++o) {
+      for (int m = 0; m < 6; ++m) {
+        double acc = 0;
+        for (int j = 0; j < i; ++j)
+          if (T.a[i][j] != 0.0) acc += T.a[i][j] * k[((size_t)j * B + o) * 6 + m];
+        Y[o * 6 + m] = y[o * 6 + m] + acc;
+      }
+      F(ti, &Y[o * 6], fb.data());
+      for (int m = 0; m < 6; ++m) k[((size_t)i * B + o) * 6 + m] = fb[m] * h;
+    }
+  }
+  for (int64_t o = 0; o < B; ++o)
+    for (int m = 0; m < 6; ++m) {
+      double sb = 0, se = 0;
+      for (int i = 0; i < s; ++i) {
+        double ki = k[((size_t)i * B + o) * 6 + m];
+        if (T.b[i] != 0.0) sb += T.b[i] * ki;
+        if (T.be[i] != 0.0) se += T.be[i] * ki;
+      }
+      y1[o * 6 + m] = y[o * 6 + m] + sb;
+      if (err) err[o * 6 + m] = se;
+    }
+}
+
+double rms(const double* x, int64_t n) {
+  double s = 0;
+  for (int64_t i = 0; i < n; ++i) s += x[i] * x[i];
+  return std::sqrt(s / (double)n);
+}
+
+bool all_finite(const double* x, int64_t n) {
+  for (int64_t i = 0; i < n; ++i)
+    if (!std::isfinite(x[i])) return false;
+  return true;
+}
+
+// diffrax _select_initial_step (Hairer, Norsett & Wanner, II.4: the final
+// exponent is 1/(p+1) for a method of order p).
+double initial_step(const Field& F, double tau0, const double* y0, int64_t n6,
+                    double rtol, double atol, double error_order) {
+  std::vector<double> f0(n6), f1(n6), y1(n6), tmp(n6), scale(n6);
+  for (int64_t i = 0; i < n6; i += 6) F(tau0, y0 + i, &f0[i]);
+  for (int64_t i = 0; i < n6; ++i) scale[i] = atol + std::fabs(y0[i]) * rtol;
+  for (int64_t i = 0; i < n6; ++i) tmp[i] = y0[i] / scale[i];
+  double d0 = rms(tmp.data(), n6);
+  for (int64_t i = 0; i < n6; ++i) tmp[i] = f0[i] / scale[i];
+  double d1 = rms(tmp.data(), n6);
+  bool cond = (d0 < 1e-5) || (d1 < 1e-5);
+  double d1_ = cond ? 1.0 : d1;
+  double h0 = cond ? 1e-6 : 0.01 * (d0 / d1_);
+  for (int64_t i = 0; i < n6; ++i) y1[i] = y0[i] + f0[i] * h0;
+  for (int64_t i = 0; i < n6; i += 6) F(tau0 + h0, &y1[i], &f1[i]);
+  for (int64_t i = 0; i < n6; ++i) tmp[i] = (f1[i] - f0[i]) / scale[i];
+  double d2 = rms(tmp.data(), n6) / h0;
+  double maxd = std::max(d1, d2);
+  double h1 = (maxd <= 1e-15) ? std::max(1e-6, h0 * 1e-3)
+                              : std::pow(0.01 / maxd, 1.0 / (error_order + 1.0));
+  return std::min(100 * h0, h1);
+}
+
+// diffrax _clip_to_end: avoid a sliver step before t1.
+double clip_to_end(double tprev, double tnext, double t1, bool keep) {
+  const double tol = 1e-10;
+  if (tnext > t1 - tol) return keep ? t1 : tprev + 0.5 * (t1 - tprev);
+  return tnext;
+}
+
+struct SolveOut {
+  int32_t status = GX_SUCCESS;
+  int64_t nsteps = 0;
+  int64_t naccept = 0;
+};
+
+// Solve B orbits that share one step sequence (B == 1 is the independent
+// per-orbit solve).  y0[6B]; save times ts[T] in t; out[B][T][6].
+SolveOut solve(const Pot& P, const gx_solver_cfg& cfg, const double* y0,
+               double t0, double t1, const double* ts, int T, int64_t B,
+               double* out) {
+  SolveOut R;
+  const Tableau& Tb = tableau(cfg.method);
+  const int64_t n6 = 6 * B;
+  auto write = [&](int s, const double* y) {
+    for (int64_t o = 0; o < B; ++o)
+      for (int m = 0; m < 6; ++m)
+        out[((size_t)o * T + s) * 6 + m] = y[o * 6 + m];
+  };
+  std::vector<double> y(y0, y0 + n6), y1(n6), err(n6), ys(n6);
+  if (t0 == t1) {
+    for (int s = 0; s < T; ++s) write(s, y.data());
+    return R;
+  }
+  const double dir = (t0 < t1) ? 1.0 : -1.0;
+  Field F{&P, dir};
+  const double tau1 = dir * t1;
+  double tau = dir * t0;
+  int si = 0;
+  // saves at (or before) the start time
+  while (si < T && dir * ts[si] <= tau) write(si++, y.data());
+
+  const bool pid = cfg.controller == GX_PID;
+  const double dtmax = (cfg.dtmax > 0) ? cfg.dtmax : INFINITY;
+  const double dtmin = (cfg.dtmin > 0) ? cfg.dtmin : 0.0;
+  double h0;
+  bool at_dtmin = false;
+  if (cfg.dt0 > 0) {
+    h0 = cfg.dt0;
+  } else {
+    h0 = initial_step(F, tau, y.data(), n6, cfg.rtol, cfg.atol, Tb.order);
+  }
+  if (pid) {
+    h0 = std::min(h0, dtmax);
+    if (dtmin > 0) {
+      at_dtmin = h0 <= dtmin;
+      h0 = std::max(h0, dtmin);
+    }
+  }
+  double tnext = std::min(tau + h0, tau1);
+  double prev_inv = 1.0, prev_prev_inv = 1.0;
+  const double order = Tb.order;
+  const double c1 = (cfg.icoeff + cfg.pcoeff + cfg.dcoeff) / order;
+  const double c2 = -(cfg.pcoeff + 2 * cfg.dcoeff) / order;
+  const double c3 = cfg.dcoeff / order;
+
+  // "unbounded" (max_steps <= 0) is a safety cap of 10,000,000 attempts,
+  // as in the GPU library
+  const int64_t cap =
+      (cfg.max_steps > 0 && cfg.max_steps < 10000000) ? cfg.max_steps : 10000000;
+  while (tau < tau1) {
+    if (R.nsteps >= cap) {
+      R.status = GX_MAX_STEPS_REACHED;
+      break;
+    }
+    const double h = tnext - tau;
+    rk_step(Tb, F, tau, y.data(), h, B, y1.data(), err.data());
+    R.nsteps++;
+    if (!all_finite(y1.data(), n6)) {
+      R.status = GX_NONFINITE;
+      break;
+    }
+    bool keep = true;
+    double next_tau1;
+    if (pid) {
+      std::vector<double> sc(n6);
+      for (int64_t i = 0; i < n6; ++i) {
+        double ym = std::max(std::fabs(y[i]), std::fabs(y1[i]));
+        sc[i] = err[i] / (cfg.atol + ym * cfg.rtol);
+      }
+      double scaled = rms(sc.data(), n6);
+      keep = scaled < 1.0;
+      if (dtmin > 0) keep = keep || at_dtmin;
+      double inv = 1.0 / scaled;
+      double f1 = (c1 == 0) ? 1.0 : std::pow(inv, c1);
+      double f2 = (c2 == 0) ? 1.0 : std::pow(prev_inv, c2);
+      double f3 = (c3 == 0) ? 1.0 : std::pow(prev_prev_inv, c3);
+      double fmin = keep ? 1.0 : cfg.factormin;
+      double factor =
+          std::min(std::max(cfg.safety * f1 * f2 * f3, fmin), cfg.factormax);
+      double dt = h * factor;
+      dt = std::min(dt, dtmax);
+      if (dtmin > 0) {
+        if (!cfg.force_dtmin && dt < dtmin) {
+          R.status = GX_DT_MIN_REACHED;
+        }
+        at_dtmin = dt <= dtmin;
+        dt = std::max(dt, dtmin);
+      }
+      if (keep) {
+        prev_prev_inv = prev_inv;
+        prev_inv = (inv == 0.0 || std::isinf(inv)) ? 1.0 : inv;
+      }
+      double nt0 = keep ? tnext : tau;
+      next_tau1 = nt0 + dt;
+    } else {
+      next_tau1 = tnext + cfg.dt0;
+    }
+    if (keep) {
+      // saves in (tau, tnext]
+      while (si < T && dir * ts[si] <= tnext) {
+        double ts_tau = dir * ts[si];
+        if (ts_tau == tnext) {
+          write(si, y1.data());
+        } else {
+          rk_step(Tb, F, tau, y.data(), ts_tau - tau, B, ys.data(), nullptr);
+          write(si, ys.data());
+        }
+        ++si;
+      }
+      y.swap(y1);
+      R.naccept++;
+    }
+    double tprev = std::min(keep ? tnext : tau, tau1);
+    tnext = clip_to_end(tprev, next_tau1, tau1, keep);
+    tau = tprev;
+    if (R.status == GX_DT_MIN_REACHED) break;
+  }
+  // any remaining saves (only if stopped early) get NaN
+  if (R.status != GX_SUCCESS) {
+    std::vector<double> nanv(n6, NAN);
+    while (si < T) write(si++, nanv.data());
+  }
+  return R;
+}
+
+Pot make_pot(const gx_component* comps, int ncomp, const gx_track* tracks,
+             int ntracks) {
+  Pot P;
+  P.comps.assign(comps, comps + ncomp);
+  for (int i = 0; i < ntracks; ++i) {
+    Track t;
+    t.n = tracks[i].n;
+    t.t = tracks[i].t;
+    t.q = tracks[i].q;
+    t.v = tracks[i].v;
+    P.tracks.push_back(t);
+  }
+  return P;
+}
+
+}  // namespace
+
+extern "C" {
+
+// what: 0 = Phi, 1 = grad Phi, 2 = Hessian (row-major 3x3), 3 = -grad Phi
+int orc_eval(const gx_component* comps, int ncomp, const gx_track* tracks,
+             int ntracks, int what, const double* xyz, const double* t,
+             int t_stride, int64_t n, double* out) {
+  if (!comps && ncomp > 0) return GX_EINVAL;
+  Pot P = make_pot(comps, ncomp, tracks, ntracks);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    double ti = t[t_stride ? i : 0];
+    const double* q = xyz + 3 * i;
+    if (what == 0) {
+      out[i] = potential(P, q, ti);
+    } else if (what == 1 || what == 3) {
+      double g[3];
+      gradient(P, q, ti, g);
+      double s = (what == 3) ? -1.0 : 1.0;
+      for (int k = 0; k < 3; ++k) out[3 * i + k] = s * g[k];
+    } else {
+      hessian(P, q, ti, out + 9 * i);
+    }
+  }
+  return GX_OK;
+}
+
+// Tableau export: fills c[s], a[s*s] (row-major), b[s], be[s]; returns s.
+int orc_tableau(int method, double* c, double* a, double* b, double* be,
+                int* order) {
+  const Tableau& T = tableau(method);
+  for (int i = 0; i < T.s; ++i) {
+    c[i] = T.c[i];
+    b[i] = T.b[i];
+    be[i] = T.be[i];
+    for (int j = 0; j < T.s; ++j) a[i * T.s + j] = T.a[i][j];
+  }
+  *order = T.order;
+  return T.s;
+}
+
+// One step of size h from (t, y[6B]) for B orbits (the OrbitSolver.step
+// doctest pins this, dynamics/_src/orbit/solver.py:300-321).
+int orc_rk_step(const gx_component* comps, int ncomp, const gx_track* tracks,
+                int ntracks, int method, double t, const double* y, double h,
+                int64_t B, double* y1, double* err) {
+  Pot P = make_pot(comps, ncomp, tracks, ntracks);
+  Field F{&P, 1.0};
+  rk_step(tableau(method), F, t, y, h, B, y1, err);
+  return GX_OK;
+}
+
+// Integrate n orbits.  shared != 0: one solve with a shared step sequence
+// over all n (t0[0], t1[0], shared ts).  Otherwise independent solves,
+// parallel over orbits with nthreads OpenMP threads (0 = default).
+int orc_integrate(const gx_component* comps, int ncomp, const gx_track* tracks,
+                  int ntracks, const gx_solver_cfg* cfg, const double* w0,
+                  const double* t0, const double* t1, int t_stride,
+                  const double* ts, int T, int ts_mode, int64_t n, int shared,
+                  int nthreads, double* out, int32_t* status, int64_t* nsteps,
+                  int64_t* naccept) {
+  if (!cfg || !w0 || !t0 || !t1 || !out || n < 0 || T < 1) return GX_EINVAL;
+  if (cfg->controller == GX_CONSTANT && !(cfg->dt0 > 0)) return GX_EINVAL;
+  if (ts_mode == GX_TS_FINAL && T != 1) return GX_EINVAL;
+  if ((ts_mode == GX_TS_SHARED || ts_mode == GX_TS_PER_ORBIT) && !ts)
+    return GX_EINVAL;
+  Pot P = make_pot(comps, ncomp, tracks, ntracks);
+  if (shared) {
+    if (ts_mode == GX_TS_PER_ORBIT) return GX_EINVAL;
+    std::vector<double> tsv(T);
+    for (int s = 0; s < T; ++s) {
+      if (ts_mode == GX_TS_SHARED)
+        tsv[s] = ts[s];
+      else if (ts_mode == GX_TS_FINAL)
+        tsv[s] = t1[0];
+      else
+        tsv[s] = (s == T - 1) ? t1[0]
+                              : t0[0] + s * ((t1[0] - t0[0]) / (T - 1));
+    }
+    SolveOut R = solve(P, *cfg, w0, t0[0], t1[0], tsv.data(), T, n, out);
+    for (int64_t i = 0; i < n; ++i) {
+      if (status) status[i] = R.status;
+      if (nsteps) nsteps[i] = R.nsteps;
+      if (naccept) naccept[i] = R.naccept;
+    }
+    return GX_OK;
+  }
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    std::vector<double> tsv(T);
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t i = 0; i < n; ++i) {
+      double a = t0[t_stride ? i : 0], b = t1[t_stride ? i : 0];
+      for (int s = 0; s < T; ++s) {
+        if (ts_mode == GX_TS_SHARED)
+          tsv[s] = ts[s];
+        else if (ts_mode == GX_TS_PER_ORBIT)
+          tsv[s] = ts[(size_t)i * T + s];
+        else if (ts_mode == GX_TS_FINAL)
+          tsv[s] = b;
+        else
+          tsv[s] = (s == T - 1) ? b : a + s * ((b - a) / (T - 1));
+      }
+      SolveOut R =
+          solve(P, *cfg, w0 + 6 * i, a, b, tsv.data(), T, 1, out + (size_t)i * T * 6);
+      if (status) status[i] = R.status;
+      if (nsteps) nsteps[i] = R.nsteps;
+      if (naccept) naccept[i] = R.naccept;
+    }
+  }
+  return GX_OK;
+}
+
+}  // extern "C"

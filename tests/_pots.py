@@ -1,0 +1,77 @@
+"""Potentials used across tests, built from the golden-vector descriptions."""
+
+import numpy as np
+
+import galax_amd.potential as gp
+from galax_amd.units import KMS, Q
+
+
+def from_spec(cls_name: str, params: dict):
+    cls = getattr(gp, cls_name)
+    if cls_name == "MonariEtAl2016BarPotential":
+        return cls(
+            alpha=params["alpha"],
+            R0=params["R0"],
+            v0=Q(params["v0_kms"], "km/s"),
+            Rb=params["Rb"],
+            phi_b=Q(params["phi_b_deg"], "deg"),
+            Omega=Q(params["Omega_kms_kpc"], "km/(s kpc)"),
+        )
+    return cls(**params)
+
+
+def ic(entry):
+    q0 = np.atleast_2d(np.asarray(entry["q0"], dtype=float))
+    if "p0_kms" in entry:
+        p0 = np.atleast_2d(np.asarray(entry["p0_kms"], dtype=float)) * KMS
+    else:
+        p0 = np.atleast_2d(np.asarray(entry["p0"], dtype=float))
+    q0, p0 = np.broadcast_arrays(q0, p0)
+    t1 = np.asarray(entry["t1"], dtype=float)
+    if t1.ndim:
+        q0 = np.broadcast_to(q0, (t1.size, 3))
+        p0 = np.broadcast_to(p0, (t1.size, 3))
+    return np.concatenate([q0, p0], axis=1), t1
+
+
+def solver_from(spec):
+    from galax_amd.dynamics import Dopri5, Dopri8, OrbitSolver, PIDController
+
+    kw = dict(rtol=spec["rtol"], atol=spec["atol"])
+    if "dtmin" in spec:
+        kw.update(dtmin=spec["dtmin"], force_dtmin=spec.get("force_dtmin", True))
+    meth = Dopri5() if spec.get("method") == "dopri5" else Dopri8()
+    return OrbitSolver(meth, stepsize_controller=PIDController(**kw), max_steps=spec.get("max_steps"))
+
+
+def zoo():
+    """A spread of models and frames exercising every evaluator path."""
+    mw = gp.MilkyWayPotential()
+    bar = gp.MonariEtAl2016BarPotential(alpha=0.01, R0=8.0, v0=Q(220.0, "km/s"), Rb=3.5,
+                                        phi_b=Q(25.0, "deg"), Omega=Q(40.0, "km/(s kpc)"))
+    om = bar.Omega
+    subs = {
+        f"sub{i}": gp.TranslatedPotential(
+            gp.PlummerPotential(m_tot=10 ** (6 + i), r_s=0.2 + 0.1 * i),
+            gp.RotatingTranslation([8.0 + i, 1.0 - i, 0.1 * i], om),
+        )
+        for i in range(3)
+    }
+    tt = np.linspace(-50.0, 50.0, 41)
+    track = gp.TimeDependentTranslationParameter(
+        tt, np.stack([10 * np.cos(0.02 * tt), 10 * np.sin(0.02 * tt), 0.5 * np.sin(0.01 * tt)], -1),
+        np.stack([-0.2 * np.sin(0.02 * tt), 0.2 * np.cos(0.02 * tt), 0.005 * np.cos(0.01 * tt)], -1),
+    )
+    return {
+        "kepler": gp.KeplerPotential(m_tot=1e11),
+        "hernquist": gp.HernquistPotential(m_tot=1e12, r_s=5.0),
+        "mn": gp.MiyamotoNagaiPotential(m_tot=6.8e10, a=3.0, b=0.28),
+        "nfw": gp.NFWPotential(m=5.4e11, r_s=15.62),
+        "plummer": gp.PlummerPotential(m_tot=1e10, r_s=1.0),
+        "bar": bar,
+        "mw": mw,
+        "mw_bar_subs": gp.CompositePotential(mw=mw, bar=bar, **subs),
+        "offset_nfw": gp.TranslatedPotential(gp.NFWPotential(m=1e11, r_s=3.0), [1.0, -2.0, 0.5]),
+        "offset_bar": gp.TranslatedPotential(bar, [0.5, 0.0, 0.0]),
+        "track_plummer": gp.TranslatedPotential(gp.PlummerPotential(m_tot=1e9, r_s=0.3), track),
+    }

@@ -1,0 +1,226 @@
+"""Parity of the HIP library with the CPU oracle on identical seeded inputs.
+
+Bars (north star): fixed-step runs agree to 1e-12 relative in q and p;
+adaptive runs to 1e-8; field evaluations to near machine precision."""
+
+import numpy as np
+import pytest
+
+import oracle
+from galax_amd import _abi
+import galax_amd.dynamics as gd
+import galax_amd.potential as gp
+from galax_amd import workloads
+from galax_amd.units import KMS
+
+from _pots import from_spec, ic, solver_from, zoo
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _rel(a, b, axis=-1):
+    """|a - b| / |b| per vector (last axis)."""
+    return np.linalg.norm(a - b, axis=axis) / np.maximum(np.linalg.norm(b, axis=axis), 1e-300)
+
+
+# ------------------------------------------------------------------ fields
+@pytest.mark.parametrize("name", list(zoo()))
+def test_field_parity(name):
+    pot = zoo()[name]
+    rng = np.random.default_rng(11)
+    n = 4096
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    x = d * np.exp(rng.uniform(np.log(0.3), np.log(60.0), n))[:, None]
+    t = rng.uniform(-500.0, 500.0, n)
+    say("field", name, n)
+    phi = pot.potential(x, t)
+    g = pot.gradient(x, t)
+    a = pot.acceleration(x, t)
+    H = pot.hessian(x, t)
+    phi_o = oracle.evaluate(pot, "potential", x, t)
+    g_o = oracle.evaluate(pot, "gradient", x, t)
+    H_o = oracle.evaluate(pot, "hessian", x, t)
+    assert np.max(np.abs(phi - phi_o) / np.abs(phi_o)) < 1e-13
+    assert np.max(_rel(g, g_o)) < 1e-12
+    np.testing.assert_array_equal(a, -g)
+    hn = np.linalg.norm(H_o.reshape(n, 9), axis=1)
+    assert np.max(np.linalg.norm((H - H_o).reshape(n, 9), axis=1) / hn) < 1e-11
+
+
+def test_field_golden(golden):
+    x = np.array(golden["point"]["x"])
+    for e in golden["potentials"]:
+        pot = from_spec(e["class"], e["params"])
+        say("golden field", e["name"])
+        np.testing.assert_allclose(pot.gradient(x[None])[0], e["grad"], atol=5.1e-9)
+        np.testing.assert_allclose(pot.hessian(x[None])[0], e["hessian"], rtol=5e-8, atol=5.1e-9)
+        if e["name"] != "milky_way":
+            assert pot.potential(x[None])[0] == pytest.approx(e["phi"], abs=5.1e-9)
+
+
+def test_field_scalar_and_batched_time_shapes():
+    pot = gp.MilkyWayPotential()
+    say("field shapes")
+    x = np.zeros((2, 5, 3)) + np.array([8.0, 0.0, 0.1])
+    assert pot.acceleration(x, 0.0).shape == (2, 5, 3)
+    assert pot.potential(x, np.zeros((2, 5))).shape == (2, 5)
+    assert pot.hessian(x[0], 1.0).shape == (5, 3, 3)
+
+
+# ------------------------------------------------------------ integration
+def test_fixed_step_plain_small_parity():
+    """1,024 MW orbits, Dopri8 fixed step 0.5 Myr, 0..1 Gyr, on-grid saves."""
+    w0, t0, t1, ts = workloads.config_plain_small()
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.ConstantStepSize(), max_steps=None)
+    say("fixed-step plain small", w0.shape, len(ts))
+    res = gd.integrate_orbits(gp.MilkyWayPotential(), w0, t0, t1, ts=ts, solver=solver, dt0=0.5)
+    out = res.out.cpu().numpy()
+    ref, st, ns, na = oracle.integrate(gp.MilkyWayPotential(), solver, w0, t0, t1, ts=ts, dt0=0.5)
+    assert np.all(res.status.cpu().numpy() == 0)
+    np.testing.assert_array_equal(res.nsteps.cpu().numpy(), ns)
+    assert np.all(ns == 2000)
+    eq = _rel(out[..., :3], ref[..., :3])
+    ep = _rel(out[..., 3:], ref[..., 3:])
+    print("max rel q", eq.max(), "p", ep.max(), flush=True)
+    assert eq.max() < 1e-12 and ep.max() < 1e-12
+
+
+def test_adaptive_uneven_parity():
+    """Per-orbit end times, rtol 1e-8 / atol 1e-10, per-orbit linspace saves."""
+    w0, t0, T = workloads.config_uneven(n=512, seed=5)
+    T = np.minimum(T, 600.0)
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.PIDController(rtol=1e-8, atol=1e-10), max_steps=None)
+    say("adaptive uneven", w0.shape)
+    res = gd.integrate_orbits(gp.MilkyWayPotential(), w0, t0, T, T=16, solver=solver)
+    out = res.out.cpu().numpy()
+    ref, st, ns, na = oracle.integrate(gp.MilkyWayPotential(), solver, w0, t0, T, T=16)
+    assert np.all(res.status.cpu().numpy() == 0)
+    same = res.nsteps.cpu().numpy() == ns
+    print("identical step counts:", same.mean(), flush=True)
+    assert same.mean() > 0.95
+    eq = _rel(out[..., :3], ref[..., :3])
+    ep = _rel(out[..., 3:], ref[..., 3:])
+    print("max rel q", eq.max(), "p", ep.max(), flush=True)
+    assert eq.max() < 1e-8 and ep.max() < 1e-8
+
+
+@pytest.mark.parametrize("key", ["solve_single_pid1e8", "solve_single_raw_array_pid1e8", "solve_batched_t1_independent_pid1e8"])
+def test_adaptive_doctests_on_gpu(golden, key):
+    e = golden["integrator"][key]
+    pot = from_spec(e["potential"]["class"], e["potential"]["params"])
+    w0, t1 = ic(e)
+    say("doctest", key)
+    res = gd.integrate_orbits(pot, w0, e["t0"], t1, solver=solver_from(e["solver"]))
+    out = res.out.cpu().numpy()
+    np.testing.assert_allclose(out[:, 0, :3], np.array(e["q"]).reshape(-1, 3), atol=5.5e-9)
+    np.testing.assert_allclose(out[:, 0, 3:], np.array(e["p"]).reshape(-1, 3), atol=5.5e-9)
+
+
+def test_one_step_doctest_on_gpu(golden):
+    e = golden["integrator"]["one_step_dopri8"]
+    pot = from_spec(e["potential"]["class"], e["potential"]["params"])
+    q0 = np.array(e["q0"])
+    w0 = np.concatenate([q0, np.broadcast_to(np.array(e["p0_kms"]) * KMS, q0.shape)], 1)
+    say("one step doctest")
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.ConstantStepSize(), max_steps=None)
+    res = gd.integrate_orbits(pot, w0, 0.0, 10.0, solver=solver, dt0=10.0)
+    out = res.out.cpu().numpy()
+    assert np.all(res.nsteps.cpu().numpy() == 1)
+    np.testing.assert_allclose(out[:, 0, :3], e["q"], atol=5e-9)
+    np.testing.assert_allclose(out[:, 0, 3:], e["p"], atol=5e-9)
+
+
+@pytest.mark.parametrize("name", ["mw_bar_subs", "track_plummer", "offset_bar", "bar"])
+def test_adaptive_parity_moving_potentials(name):
+    pot = zoo()[name]
+    w0 = workloads.disk_ics(128, 21, R_range=(3.0, 15.0))
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.PIDController(rtol=1e-8, atol=1e-10), max_steps=None)
+    say("moving potential", name)
+    res = gd.integrate_orbits(pot, w0, -40.0, 300.0, T=8, solver=solver)
+    ref, st, ns, na = oracle.integrate(pot, solver, w0, -40.0, 300.0, T=8)
+    out = res.out.cpu().numpy()
+    assert np.all(res.status.cpu().numpy() == 0)
+    assert np.max(_rel(out[..., :3], ref[..., :3])) < 1e-8
+    assert np.max(_rel(out[..., 3:], ref[..., 3:])) < 1e-8
+
+
+def test_dopri5_parity():
+    w0 = workloads.disk_ics(256, 31)
+    say("dopri5")
+    for ctl, kw in [(gd.ConstantStepSize(), {"dt0": 0.25}), (gd.PIDController(rtol=1e-9, atol=1e-9), {})]:
+        solver = gd.OrbitSolver(gd.Dopri5(), stepsize_controller=ctl, max_steps=None)
+        res = gd.integrate_orbits(gp.MilkyWayPotential(), w0, 0.0, 250.0, T=5, solver=solver, **kw)
+        ref, *_ = oracle.integrate(gp.MilkyWayPotential(), solver, w0, 0.0, 250.0, T=5, **kw)
+        tol = 1e-12 if isinstance(ctl, gd.ConstantStepSize) else 1e-8
+        assert np.max(_rel(res.out.cpu().numpy()[..., :3], ref[..., :3])) < tol
+
+
+def test_save_modes_and_backward():
+    pot = gp.MilkyWayPotential()
+    w0 = workloads.disk_ics(64, 41)
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.ConstantStepSize(), max_steps=None)
+    say("save modes")
+    rng = np.random.default_rng(0)
+    ts = np.sort(rng.uniform(0.0, 100.0, (64, 7)), axis=1)
+    ts[:, 0], ts[:, -1] = 0.0, 100.0
+    res = gd.integrate_orbits(pot, w0, 0.0, 100.0, ts=ts, solver=solver, dt0=0.7)
+    ref, *_ = oracle.integrate(pot, solver, w0, 0.0, 100.0, ts=ts, dt0=0.7)
+    assert np.max(_rel(res.out.cpu().numpy()[..., :3], ref[..., :3])) < 1e-12
+    # backward: -> t = -100 and back
+    back = gd.integrate_orbits(pot, w0, 0.0, -100.0, solver=solver, dt0=0.5)
+    ref_b, *_ = oracle.integrate(pot, solver, w0, 0.0, -100.0, dt0=0.5)
+    assert np.max(_rel(back.out.cpu().numpy()[:, 0, :3], ref_b[:, 0, :3])) < 1e-12
+    fwd = gd.integrate_orbits(pot, back.out[:, 0, :], -100.0, 0.0, solver=solver, dt0=0.5)
+    np.testing.assert_allclose(fwd.out.cpu().numpy()[:, 0, :], w0, atol=1e-10)
+
+
+def test_statuses_and_edges():
+    pot = gp.HernquistPotential(m_tot=1e12, r_s=5.0)
+    w0 = np.array([[10.0, 0.0, 0.0, 0.0, 0.2, 0.0], [np.nan, 0, 0, 0, 0.2, 0], [10.0, 0, 0, 0, 0.2, 0]])
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.PIDController(rtol=1e-8, atol=1e-8), max_steps=5)
+    say("statuses")
+    res = gd.integrate_orbits(pot, w0, np.array([0.0, 0.0, 50.0]), np.array([1000.0, 1000.0, 50.0]),
+                              ts=np.tile(np.array([0.0, 500.0, 1000.0]), (3, 1)), solver=solver)
+    st = res.status.cpu().numpy()
+    out = res.out.cpu().numpy()
+    assert st[0] == _abi.GX_MAX_STEPS_REACHED and res.nsteps.cpu().numpy()[0] == 5
+    assert np.all(np.isnan(out[0, 1:]))
+    assert st[1] == _abi.GX_NONFINITE
+    assert st[2] == _abi.GX_SUCCESS and res.nsteps.cpu().numpy()[2] == 0
+    np.testing.assert_array_equal(out[2, 0], w0[2])
+    empty = gd.integrate_orbits(pot, np.zeros((0, 6)), 0.0, 1.0, solver=solver)
+    assert tuple(empty.out.shape) == (0, 1, 6)
+
+
+def test_queue_order_does_not_change_results():
+    w0, t0, T = workloads.config_uneven(n=2048, seed=9)
+    T = np.minimum(T, 300.0)
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.PIDController(rtol=1e-8, atol=1e-10), max_steps=None)
+    say("queue order")
+    a = gd.integrate_orbits(gp.MilkyWayPotential(), w0, t0, T, T=4, solver=solver)
+    b = gd.integrate_orbits(gp.MilkyWayPotential(), w0, t0, T, T=4, solver=solver, order=workloads.cost_order(w0, T))
+    np.testing.assert_array_equal(a.out.cpu().numpy(), b.out.cpu().numpy())
+
+
+def test_evaluate_orbit_two_stage():
+    pot = gp.KeplerPotential(m_tot=1e12)
+    say("evaluate_orbit two stage")
+    w0 = gd.PhaseSpaceCoordinate(q=np.array([[10.0, 0, 0], [10.0, 0, 0]]), p=np.array([[0, 200 * KMS, 0], [0, 220 * KMS, 0]]),
+                                 t=np.array([-100.0, -150.0]))
+    ts = np.linspace(0, 1000, 10)
+    orbit = gd.evaluate_orbit(pot, w0, ts)
+    assert orbit.q.shape == (2, 10, 3) and orbit.t.shape == (10,)
+    integ = gd.Integrator()
+    solver = integ.dynamics_solver
+    w = w0.w()
+    s1, *_ = oracle.integrate(pot, solver, w, np.array([-100.0, -150.0]), 0.0, max_steps=None)
+    s2, *_ = oracle.integrate(pot, solver, s1[:, 0], 0.0, 1000.0, ts=ts, max_steps=None)
+    assert np.max(_rel(orbit.q, s2[..., :3])) < 1e-8
+    # a single time
+    o1 = gd.evaluate_orbit(pot, w0, 500.0)
+    assert o1.q.shape == (2, 1, 3)

@@ -1,0 +1,144 @@
+"""Host-side logic and the C ABI, without a GPU: structure layouts against
+the C header, the library's exported symbols, potential flattening, units,
+solver configuration and workload generation."""
+
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from galax_amd import _abi, _lib
+from galax_amd.dynamics import ConstantStepSize, Dopri5, Dopri8, OrbitSolver, PIDController
+import galax_amd.potential as gp
+from galax_amd.units import G, KMS, Q
+from galax_amd import workloads
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "galax_amd.h"
+
+
+def _declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^(?:int|int64_t)\s+(gx_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libgalaxhip.so not built")
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    names = _declared_functions()
+    assert len(names) >= 10
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_header(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "galax_amd.h"\n'
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(gx_component),"
+        " offsetof(gx_component,p), offsetof(gx_component,f), sizeof(gx_track), sizeof(gx_solver_cfg),"
+        " offsetof(gx_solver_cfg,dt0), offsetof(gx_solver_cfg,force_dtmin)); return 0;}\n"
+    )
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [
+        ctypes.sizeof(_abi.GxComponent),
+        _abi.GxComponent.p.offset,
+        _abi.GxComponent.f.offset,
+        ctypes.sizeof(_abi.GxTrack),
+        ctypes.sizeof(_abi.GxSolverCfg),
+        _abi.GxSolverCfg.dt0.offset,
+        _abi.GxSolverCfg.force_dtmin.offset,
+    ]
+    assert got == want
+
+
+def test_units_and_constants():
+    # G from astropy 7.1 constants in kpc^3 / (Msun Myr^2)
+    Gsi, msun, kpc, myr = 6.6743e-11, 1.988409870698051e30, 3.0856775814913673e19, 3.15576e13
+    assert G == pytest.approx(Gsi * msun * myr**2 / kpc**3, rel=1e-15)
+    assert KMS == pytest.approx(1e3 * myr / kpc, rel=1e-15)
+    # 220 km/s as printed by the reference (orbit/solver.py:310)
+    assert Q(220.0, "km/s").to_value() == pytest.approx(0.22499668, abs=5e-9)
+    assert Q(1.0, "Gyr").to_value("Myr") == 1000.0
+    assert Q(70, "pc").to_value() == pytest.approx(0.07)
+    with pytest.raises(ValueError):
+        Q(1.0, "furlong")
+
+
+def test_milky_way_table():
+    comps, tracks = gp.MilkyWayPotential()._table()
+    kinds = [c.kind for c in comps]
+    assert kinds == [_abi.GX_MIYAMOTO_NAGAI, _abi.GX_NFW, _abi.GX_HERNQUIST, _abi.GX_HERNQUIST]
+    assert comps[0].p[0] == G * 6.8e10 and comps[0].p[1] == 3.0 and comps[0].p[2] == 0.28
+    assert comps[1].p[0] == G * 5.4e11 and comps[1].p[1] == 15.62
+    assert comps[3].p[1] == 0.07
+    assert tracks == []
+    mw2 = gp.MilkyWayPotential(halo={"m": 1e12})
+    assert mw2._table()[0][1].p[0] == G * 1e12 and mw2._table()[0][1].p[1] == 15.62
+
+
+def test_composite_and_translation_tables():
+    bar = gp.MonariEtAl2016BarPotential(R0=8.0, v0=Q(220.0, "km/s"))
+    assert bar.Omega == pytest.approx(52.2 * KMS)
+    assert bar.phi_b == pytest.approx(np.deg2rad(25.0))
+    tr = gp.TimeDependentTranslationParameter(np.linspace(0, 10, 11), np.zeros((11, 3)))
+    pot = gp.CompositePotential(
+        a=gp.KeplerPotential(m_tot=1.0),
+        b=gp.TranslatedPotential(gp.PlummerPotential(m_tot=2.0, r_s=0.1), [1.0, 2.0, 3.0]),
+        c=gp.TranslatedPotential(gp.PlummerPotential(m_tot=3.0, r_s=0.1), gp.RotatingTranslation([5.0, 0, 0], 0.04)),
+        d=gp.TranslatedPotential(gp.PlummerPotential(m_tot=4.0, r_s=0.1), tr),
+        e=bar,
+    )
+    comps, tracks = pot._table()
+    assert [c.frame for c in comps] == [
+        _abi.GX_FRAME_NONE, _abi.GX_FRAME_OFFSET, _abi.GX_FRAME_ROTATING, _abi.GX_FRAME_TRACK, _abi.GX_FRAME_NONE]
+    assert list(comps[1].f[:3]) == [1.0, 2.0, 3.0]
+    assert comps[2].f[3] == 0.04
+    assert comps[3].track == 0 and len(tracks) == 1
+    with pytest.raises(ValueError):
+        gp.TranslatedPotential(pot["b"], [0.0, 0.0, 1.0])._rows()
+    s = gp.KeplerPotential(m_tot=1.0) + gp.HernquistPotential(m_tot=1.0, r_s=1.0)
+    assert len(s) == 2
+
+
+def test_solver_configuration():
+    c = OrbitSolver().cfg()
+    assert (c.method, c.controller, c.max_steps) == (_abi.GX_DOPRI8, _abi.GX_PID, 2**16)
+    assert (c.rtol, c.atol, c.icoeff, c.pcoeff, c.dcoeff) == (1e-8, 1e-8, 1.0, 0.0, 0.0)
+    assert (c.safety, c.factormin, c.factormax, c.dt0) == (0.9, 0.2, 10.0, 0.0)
+    c = OrbitSolver(Dopri5(), stepsize_controller=ConstantStepSize(), max_steps=None).cfg(dt0=-0.5)
+    assert (c.method, c.controller, c.dt0, c.max_steps) == (_abi.GX_DOPRI5, _abi.GX_CONSTANT, 0.5, 0)
+    with pytest.raises(ValueError):
+        OrbitSolver(stepsize_controller=ConstantStepSize()).cfg()
+    c = OrbitSolver(stepsize_controller=PIDController(rtol=1e-6, atol=1e-9, dtmin=0.3)).cfg(dt0=2.0)
+    assert (c.dtmin, c.dt0, c.force_dtmin) == (0.3, 2.0, 1)
+
+
+def test_workloads_are_seeded_and_shaped():
+    w0, t0, t1, ts = workloads.config_plain_small()
+    assert w0.shape == (1024, 6) and ts[0] == 0.0 and ts[-1] == 1000.0 and len(ts) == 101
+    np.testing.assert_array_equal(w0, workloads.config_plain_small()[0])
+    R = np.hypot(w0[:, 0], w0[:, 1])
+    assert R.min() >= 4.0 and R.max() <= 20.0
+    # speeds scatter around the circular speed (10% dispersion)
+    assert 0.15 < np.median(np.linalg.norm(w0[:, 3:], axis=1)) < 0.3
+    w, t0, T = workloads.config_uneven(n=4096)
+    assert T.min() >= 100.0 and T.max() <= 5000.0
+    order = workloads.cost_order(w, T)
+    assert sorted(order.tolist()) == list(range(4096))
+
+
+def test_numpy_gradient_matches_oracle():
+    import oracle
+
+    rng = np.random.default_rng(3)
+    x = rng.normal(0, 10, (64, 3))
+    np.testing.assert_allclose(
+        workloads.mw_gradient_np(x), oracle.evaluate(gp.MilkyWayPotential(), "gradient", x), rtol=1e-12
+    )
