@@ -58,6 +58,14 @@ def main() -> int:
     ap.add_argument("--saves", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument(
+        "--workload",
+        choices=("uneven", "uniform", "longest"),
+        default="uneven",
+        help="uneven: the benchmark configuration; uniform: the same orbits with every "
+        "integration time set to 200 Myr (balanced work: chip throughput); longest: the "
+        "single most expensive orbit of the configuration alone (per-step latency)",
+    )
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -79,6 +87,13 @@ def main() -> int:
     n = args.n
     w0, t0, T = workloads.config_uneven(n=n, seed=2)
     order = workloads.cost_order(w0, T)
+    if args.workload == "uniform":
+        T = np.full(n, 200.0)
+    elif args.workload == "longest":
+        sel = order[:1]
+        w0, t0, T = w0[sel], t0[sel], T[sel]
+        n = 1
+        order = np.zeros(1, dtype=np.int64)
     solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.PIDController(rtol=1e-8, atol=1e-10), max_steps=None)
 
     w0d = torch.as_tensor(w0, device="cuda")
@@ -136,6 +151,8 @@ def main() -> int:
             "accepted_steps": int(naccept.sum()),
             "failed_orbits": int((status != 0).sum()),
             "wall_ms_per_step": wall * 1e3,
+            "max_steps_one_orbit": int(nsteps.max()),
+            "diagnostic_workload": args.workload,
         },
         "roofline": {
             "bound": "mfma",

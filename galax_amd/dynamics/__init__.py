@@ -10,6 +10,14 @@ from ._orbit import (
     evaluate_orbit,
     parse_w0,
 )
+from ._mockstream import (
+    FardalStreamDF,
+    MockStream,
+    MockStreamArm,
+    MockStreamGenerator,
+    omega,
+    tidal_radius,
+)
 from ._solver import (
     ConstantStepSize,
     Dopri5,
@@ -21,6 +29,12 @@ from ._solver import (
 )
 
 __all__ = [
+    "FardalStreamDF",
+    "MockStream",
+    "MockStreamArm",
+    "MockStreamGenerator",
+    "omega",
+    "tidal_radius",
     "HamiltonianField",
     "Integrator",
     "Orbit",
