@@ -64,7 +64,7 @@ constexpr long long kStepCap = 10000000;  // attempted steps per orbit
 // emitted once; each stage's coefficients and its slot in g[][] are selected
 // by a chain of compile-time-indexed branches on the (wave-uniform) stage
 // number, which keeps g[][] in registers.
-template <class M, bool FULL, bool ERR>
+template <class M, class E, bool ERR>
 __device__ __forceinline__ void rk_eval(const DPot& P, double dir, double tb,
                                         double h, const double (&q)[3],
                                         const double (&p)[3], double (&q1)[3],
@@ -72,26 +72,26 @@ __device__ __forceinline__ void rk_eval(const DPot& P, double dir, double tb,
                                         double (&ep)[3]) {
   constexpr int S = M::S;
   double g[S][3];
-  if (!FULL) {
+  if constexpr (!E::kLarge) {
     // Small evaluator: unroll the stages; every index below is static.
-    accel<FULL>(P, q[0], q[1], q[2], dir * tb, g[0][0], g[0][1], g[0][2]);
+    E::acc(P, q[0], q[1], q[2], dir * tb, g[0][0], g[0][1], g[0][2]);
 #pragma unroll
     for (int i = 1; i < S; ++i) {
       double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
       for (int l = 0; l < i; ++l) {
-        const double a = M::k.aa[i][l];
-        if (a != 0.0) {
+        if (M::k.aa[i][l] != 0.0) {
+          const double a = M::k.aa[i][l];
           s0 = fma(a, g[l][0], s0);
           s1 = fma(a, g[l][1], s1);
           s2 = fma(a, g[l][2], s2);
         }
       }
       const double rs = M::k.rs[i];
-      accel<FULL>(P, fma(h, fma(rs, p[0], h * s0), q[0]),
-                  fma(h, fma(rs, p[1], h * s1), q[1]),
-                  fma(h, fma(rs, p[2], h * s2), q[2]),
-                  dir * fma(M::k.c[i], h, tb), g[i][0], g[i][1], g[i][2]);
+      E::acc(P, fma(h, fma(rs, p[0], h * s0), q[0]),
+             fma(h, fma(rs, p[1], h * s1), q[1]),
+             fma(h, fma(rs, p[2], h * s2), q[2]), dir * fma(M::k.c[i], h, tb),
+             g[i][0], g[i][1], g[i][2]);
     }
   } else {
     // Large evaluator: keep the stage loop rolled so that it is emitted
@@ -127,7 +127,7 @@ __device__ __forceinline__ void rk_eval(const DPot& P, double dir, double tb,
         ts = fma(c, h, tb);
       }
       double ax, ay, az;
-      accel<FULL>(P, Qx, Qy, Qz, dir * ts, ax, ay, az);
+      E::acc(P, Qx, Qy, Qz, dir * ts, ax, ay, az);
 #pragma unroll
       for (int k = 0; k < S; ++k) {
         if (k == i) {
@@ -143,25 +143,27 @@ __device__ __forceinline__ void rk_eval(const DPot& P, double dir, double tb,
   double e0 = 0.0, e1 = 0.0, e2 = 0.0, ea0 = 0.0, ea1 = 0.0, ea2 = 0.0;
 #pragma unroll
   for (int l = 0; l < S; ++l) {
-    const double b = M::k.b[l], ba = M::k.ba[l];
-    if (b != 0.0) {
+    if (M::k.b[l] != 0.0) {
+      const double b = M::k.b[l];
       b0 = fma(b, g[l][0], b0);
       b1 = fma(b, g[l][1], b1);
       b2 = fma(b, g[l][2], b2);
     }
-    if (ba != 0.0) {
+    if (M::k.ba[l] != 0.0) {
+      const double ba = M::k.ba[l];
       ba0 = fma(ba, g[l][0], ba0);
       ba1 = fma(ba, g[l][1], ba1);
       ba2 = fma(ba, g[l][2], ba2);
     }
     if (ERR) {
-      const double e = M::k.e[l], ea = M::k.ea[l];
-      if (e != 0.0) {
+      if (M::k.e[l] != 0.0) {
+        const double e = M::k.e[l];
         e0 = fma(e, g[l][0], e0);
         e1 = fma(e, g[l][1], e1);
         e2 = fma(e, g[l][2], e2);
       }
-      if (ea != 0.0) {
+      if (M::k.ea[l] != 0.0) {
+        const double ea = M::k.ea[l];
         ea0 = fma(ea, g[l][0], ea0);
         ea1 = fma(ea, g[l][1], ea1);
         ea2 = fma(ea, g[l][2], ea2);
@@ -332,7 +334,7 @@ struct Cold {
   int si[kBlock], st[kBlock], at_dtmin[kBlock];
 };
 
-template <class M, bool PID, bool FULL>
+template <class M, bool PID, class E>
 __global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
   __shared__ Cold C;
   const int L = threadIdx.x;
@@ -414,7 +416,7 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
           }
         }
         double q1[3], p1[3], eq[3], ep[3];
-        rk_eval<M, FULL, PID>(P, dir, tb, h, qb, pb, q1, p1, eq, ep);
+        rk_eval<M, E, PID>(P, dir, tb, h, qb, pb, q1, p1, eq, ep);
 
         const double tau1 = C.tau1[L];
         if (save_mode) {
@@ -546,9 +548,10 @@ __global__ void k_zero_counter(unsigned long long* c) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ULL;
 }
 
-template <class M, bool PID, bool FULL>
+template <class M, bool PID, class E>
 int launch(const gx_pot* pot, IArgs& A, hipStream_t stream) {
-  const auto kern = k_integrate<M, PID, FULL>;
+  const auto kern = k_integrate<M, PID, E>;
+  constexpr bool FULL = E::kLarge;
   const LaunchShape& ls = launch_shape();
   const int block = kBlock;
   int occ = 0;
@@ -645,19 +648,23 @@ int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
 
   hipStream_t s = (hipStream_t)stream;
   const bool pid = cfg->controller == GX_PID;
-  const bool full = !pot->basic;
+  using Full = EvalTable<true>;
+  using Basic = EvalTable<false>;
   if (cfg->method == GX_DOPRI8) {
+    if (pot->sig == kSigMilkyWay)
+      return pid ? launch<Dopri8, true, EvalMilkyWay>(pot, A, s)
+                 : launch<Dopri8, false, EvalMilkyWay>(pot, A, s);
     if (pid)
-      return full ? launch<Dopri8, true, true>(pot, A, s)
-                  : launch<Dopri8, true, false>(pot, A, s);
-    return full ? launch<Dopri8, false, true>(pot, A, s)
-                : launch<Dopri8, false, false>(pot, A, s);
+      return pot->basic ? launch<Dopri8, true, Basic>(pot, A, s)
+                        : launch<Dopri8, true, Full>(pot, A, s);
+    return pot->basic ? launch<Dopri8, false, Basic>(pot, A, s)
+                      : launch<Dopri8, false, Full>(pot, A, s);
   }
   if (pid)
-    return full ? launch<Dopri5, true, true>(pot, A, s)
-                : launch<Dopri5, true, false>(pot, A, s);
-  return full ? launch<Dopri5, false, true>(pot, A, s)
-              : launch<Dopri5, false, false>(pot, A, s);
+    return pot->basic ? launch<Dopri5, true, Basic>(pot, A, s)
+                      : launch<Dopri5, true, Full>(pot, A, s);
+  return pot->basic ? launch<Dopri5, false, Basic>(pot, A, s)
+                    : launch<Dopri5, false, Full>(pot, A, s);
 }
 
 }  // extern "C"

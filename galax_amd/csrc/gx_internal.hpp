@@ -14,7 +14,10 @@ struct gx_pot {
   bool has_track = false;
   bool has_bar = false;
   bool basic = true;  // only origin-centred Kepler/Hernquist/MN/NFW/Plummer
+  int sig = 0;        // a composition with a specialised evaluator, or 0
 };
+
+enum : int { kSigNone = 0, kSigMilkyWay = 1 };
 
 namespace gx {
 

@@ -280,6 +280,11 @@ int gx_potential_create(const gx_component* comps, int32_t ncomp,
   for (int i = 0; i < ncomp; ++i)
     if (comps[i].frame != GX_FRAME_NONE || comps[i].kind == GX_MONARI_BAR)
       h->basic = false;
+  h->sig = kSigNone;
+  if (h->basic && ncomp == 4 && comps[0].kind == GX_MIYAMOTO_NAGAI &&
+      comps[1].kind == GX_NFW && comps[2].kind == GX_HERNQUIST &&
+      comps[3].kind == GX_HERNQUIST)
+    h->sig = kSigMilkyWay;
   *out = h;
   return GX_OK;
 }

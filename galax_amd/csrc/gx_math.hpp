@@ -1,0 +1,88 @@
+// gx_math.hpp -- short FP64 primitives for the field evaluator.
+//
+// The library's log1p is a general-purpose routine (about 125 VALU
+// instructions on gfx950); the NFW halo needs log(1 + r/r_s) 13 times per
+// Dopri8 step, where it was half of the Milky Way field's instruction count.
+// gx_log1p follows the classical construction (argument reduction by the
+// binary exponent, then the atanh series of log; e.g. Tang, "Table-driven
+// implementation of the logarithm function", ACM TOMS 16 (1990), and the
+// fdlibm e_log.c derivation):
+//   1 + x = 2^k m,  m in [sqrt(1/2), sqrt(2)),  f = m - 1,  s = f / (2 + f),
+//   log(1 + f) = 2 atanh(s) = f - f^2/2 + s (f^2/2 + R(s^2)),
+//   R(z) = sum_{n>=1} 2 z^n / (2n + 1)   (|s| < 0.1716, so z < 0.0295 and ten
+//   terms leave a truncation below 2^-55 relative),
+//   log1p(x) = k ln2_hi + [k ln2_lo + log(1 + f) + e/u],
+// where u = fl(1 + x) and e = (1 + x) - u is recovered exactly by Fast2Sum.
+// ln2_hi keeps the leading 32 bits of ln 2 (so k ln2_hi is exact) and
+// ln2_lo = ln 2 - ln2_hi.  Accuracy: within 2 ulp for x > -1 (checked against
+// the library log1p in the GPU tests).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace gx {
+
+// 1/d from the hardware reciprocal estimate and two Newton-Raphson steps
+// (each squares the relative error), ending within an ulp.
+__device__ __forceinline__ double rcp(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
+// 1/sqrt(x) for finite x > 0 from the hardware estimate and two Newton
+// steps r <- r (3 - x r^2) / 2, written as r + r (1/2 - (x r / 2) r).
+__device__ __forceinline__ double rsqrt_fast(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * r;
+  double e = fma(-x * r, h, 0.5);
+  r = fma(r, e, r);
+  h = 0.5 * r;
+  e = fma(-x * r, h, 0.5);
+  return fma(r, e, r);
+}
+
+// n/d: Newton reciprocal, then one residual correction of the quotient.
+__device__ __forceinline__ double div(double n, double d) {
+  const double r = rcp(d);
+  const double q = n * r;
+  return fma(fma(-d, q, n), r, q);
+}
+
+__device__ __forceinline__ double log1p_fast(double x) {
+  constexpr double kLn2Hi = 0x1.62e42fef00000p-1;
+  constexpr double kLn2Lo = 0x1.473de6af278edp-34;
+  constexpr double kSqrtHalf = 0.70710678118654752440;
+  const double u = 1.0 + x;
+  // rounding error of u = fl(1 + x), exact (Fast2Sum with the larger first)
+  const double e = (x > 1.0) ? (1.0 - (u - x)) : (x - (u - 1.0));
+  int k;
+  double m = frexp(u, &k);  // u = m 2^k, m in [0.5, 1)
+  if (m < kSqrtHalf) {
+    m *= 2.0;
+    k -= 1;
+  }
+  const double f = m - 1.0;
+  const double s = div(f, 2.0 + f);
+  const double z = s * s;
+  double R = 2.0 / 21.0;
+  R = fma(R, z, 2.0 / 19.0);
+  R = fma(R, z, 2.0 / 17.0);
+  R = fma(R, z, 2.0 / 15.0);
+  R = fma(R, z, 2.0 / 13.0);
+  R = fma(R, z, 2.0 / 11.0);
+  R = fma(R, z, 2.0 / 9.0);
+  R = fma(R, z, 2.0 / 7.0);
+  R = fma(R, z, 2.0 / 5.0);
+  R = fma(R, z, 2.0 / 3.0);
+  R *= z;
+  const double hfsq = 0.5 * f * f;
+  const double dk = (double)k;
+  // e/u: the whole result when x is below an ulp of 1, so keep it accurate
+  const double corr = e * rcp(u);
+  return dk * kLn2Hi - ((hfsq - (s * (hfsq + R) + (dk * kLn2Lo + corr))) - f);
+}
+
+}  // namespace gx

@@ -21,7 +21,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "../../include/galax_amd.h"
+#include "gx_math.hpp"
 
 namespace gx {
 
@@ -127,16 +130,16 @@ __device__ __forceinline__ double sph_g(int kind, const double* p, double r2,
     case GX_KEPLER:
       return p[0] * ir * ir * ir;
     case GX_HERNQUIST: {
-      const double u = 1.0 / (r + p[1]);
+      const double u = rcp(r + p[1]);
       return p[0] * u * u * ir;
     }
     case GX_NFW: {
-      const double L = log1p(r * p[2]);
-      const double u = 1.0 / (r + p[1]);
+      const double L = log1p_fast(r * p[2]);
+      const double u = rcp(r + p[1]);
       return p[0] * ir * ir * (L * ir - u);
     }
     case GX_PLUMMER: {
-      const double id = rsqrt(r2 + p[1]);
+      const double id = rsqrt_fast(r2 + p[1]);
       return p[0] * id * id * id;
     }
     default:
@@ -148,10 +151,10 @@ __device__ __forceinline__ void mn_acc(const double* p, double x, double y,
                                        double z, double& ax, double& ay,
                                        double& az) {
   const double zb = z * z + p[2];
-  const double iS = rsqrt(zb);
+  const double iS = rsqrt_fast(zb);
   const double S = zb * iS;
   const double A = p[1] + S;
-  const double iD = rsqrt(x * x + y * y + A * A);
+  const double iD = rsqrt_fast(x * x + y * y + A * A);
   const double g = p[0] * iD * iD * iD;
   ax -= g * x;
   ay -= g * y;
@@ -166,7 +169,7 @@ __device__ __forceinline__ void bar_acc(const double* p, double c2, double s2,
                                         double& ax, double& ay, double& az) {
   const double R2 = x * x + y * y;
   const double r2 = R2 + z * z;
-  const double ir = rsqrt(r2);
+  const double ir = rsqrt_fast(r2);
   const double r = r2 * ir;
   const double ir2 = ir * ir;
   const double W = (x * x - y * y) * c2 + 2.0 * x * y * s2;
@@ -195,7 +198,7 @@ __device__ __forceinline__ void accel(const DPot& P, double x, double y,
   ax = ay = az = 0.0;
   // radius about the origin, shared by untranslated spherical components
   const double r2o = x * x + y * y + z * z;
-  const double iro = rsqrt(r2o + kTiny);
+  const double iro = rsqrt_fast(r2o + kTiny);
   const double ro = (r2o + kTiny) * iro;
   double gcos = 1.0, gsin = 0.0, xr = x, yr = y;
   double gax = 0.0, gay = 0.0, gaz = 0.0;
@@ -230,7 +233,7 @@ __device__ __forceinline__ void accel(const DPot& P, double x, double y,
           mn_acc(c.p, dx, dy, dz, gax, gay, gaz);
         } else {
           const double r2 = dx * dx + dy * dy + dz * dz;
-          const double ir = rsqrt(r2 + kTiny);
+          const double ir = rsqrt_fast(r2 + kTiny);
           const double r = (r2 + kTiny) * ir;
           const double g = sph_g(kind, c.p, r2, r, ir);
           gax -= g * dx;
@@ -272,7 +275,7 @@ __device__ __forceinline__ void accel(const DPot& P, double x, double y,
         bar_acc(c.p, c2, s2, dx, dy, dz, ax, ay, az);
       } else if (kind != GX_NULL) {
         const double r2 = dx * dx + dy * dy + dz * dz;
-        const double ir = rsqrt(r2 + kTiny);
+        const double ir = rsqrt_fast(r2 + kTiny);
         const double r = (r2 + kTiny) * ir;
         const double g = sph_g(kind, c.p, r2, r, ir);
         ax -= g * dx;
@@ -282,6 +285,62 @@ __device__ __forceinline__ void accel(const DPot& P, double x, double y,
     }
   }
 }
+
+// Straight-line evaluator for a fixed sequence of origin-centred models:
+// the kinds are template arguments, the parameters still come from the
+// table.  With no loop or kind dispatch, the compiler interleaves the
+// components' independent dependency chains.
+template <int K>
+__device__ __forceinline__ void comp_acc(const DComp& c, double x, double y,
+                                         double z, double r2, double r,
+                                         double ir, double& ax, double& ay,
+                                         double& az) {
+  if constexpr (K == GX_MIYAMOTO_NAGAI) {
+    mn_acc(c.p, x, y, z, ax, ay, az);
+  } else {
+    const double g = sph_g(K, c.p, r2, r, ir);
+    ax -= g * x;
+    ay -= g * y;
+    az -= g * z;
+  }
+}
+
+template <int... K>
+struct EvalFixed {
+  static constexpr bool kLarge = false;
+  template <size_t... I>
+  __device__ __forceinline__ static void run(const DPot& P, double x, double y,
+                                             double z, double& ax, double& ay,
+                                             double& az,
+                                             std::index_sequence<I...>) {
+    ax = ay = az = 0.0;
+    const double r2 = x * x + y * y + z * z;
+    const double ir = rsqrt_fast(r2 + kTiny);
+    const double r = (r2 + kTiny) * ir;
+    (comp_acc<K>(P.comps[I], x, y, z, r2, r, ir, ax, ay, az), ...);
+  }
+  __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
+                                             double z, double /*t*/,
+                                             double& ax, double& ay,
+                                             double& az) {
+    run(P, x, y, z, ax, ay, az, std::make_index_sequence<sizeof...(K)>{});
+  }
+};
+
+// Table-driven evaluators (any composition).
+template <bool FULL>
+struct EvalTable {
+  static constexpr bool kLarge = FULL;
+  __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
+                                             double z, double t, double& ax,
+                                             double& ay, double& az) {
+    accel<FULL>(P, x, y, z, t, ax, ay, az);
+  }
+};
+
+// galax's MilkyWayPotential: disk, halo, bulge, nucleus (milkyway.py:173-236)
+using EvalMilkyWay =
+    EvalFixed<GX_MIYAMOTO_NAGAI, GX_NFW, GX_HERNQUIST, GX_HERNQUIST>;
 
 // ------------------------------------------------ potential and Hessian
 // (not on the integration hot path: each component in the inertial frame)

@@ -89,4 +89,4 @@ def test_stream_release_integration_parity():
     got = np.concatenate([stream["lead"].w(), stream["trail"].w()])
     from test_gpu_parity import check_adaptive
 
-    check_adaptive(got[:, None, :], ref, stream.nsteps, _ns, all_tol=1e-5)
+    check_adaptive(got[:, None, :], ref, stream.nsteps, _ns, all_tol=1e-5, frac_tight=0.8)
