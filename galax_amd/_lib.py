@@ -41,10 +41,10 @@ def _declare(lib: ctypes.CDLL) -> None:
         "gx_device_info": (ctypes.c_int, [ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     }
     optional = {
-        "gx_adjoint_workspace_bytes": (_i64, [_i64, _i32, _i32]),
+        "gx_adjoint_workspace_bytes": (_i64, [_i64, _i64, _i32]),
         "gx_adjoint": (
             ctypes.c_int,
-            [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp],
+            [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _i64, _vp, _vp, _vp, _i32, _i64, _i32, _vp, _vp, _vp],
         ),
         "gx_mlp_forward": (ctypes.c_int, [_vp, _vp, _i64, _vp, _vp, _vp]),
         "gx_mlp_train_step": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),

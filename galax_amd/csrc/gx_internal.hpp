@@ -15,6 +15,7 @@ struct gx_pot {
   bool has_bar = false;
   bool basic = true;  // only origin-centred Kepler/Hernquist/MN/NFW/Plummer
   int sig = 0;        // a composition with a specialised evaluator, or 0
+  int kind0 = 0;      // kind of the first component
 };
 
 enum : int { kSigNone = 0, kSigMilkyWay = 1 };

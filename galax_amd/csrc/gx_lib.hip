@@ -88,6 +88,12 @@ int derive(const gx_component& in, DComp& out) {
     default:
       return GX_EINVAL;
   }
+  // raw table parameters for the adjoint (spare slots of the basic kinds)
+  if (in.kind >= GX_KEPLER && in.kind <= GX_PLUMMER) {
+    out.p[5] = p[0];
+    out.p[6] = p[1];
+    out.p[7] = p[2];
+  }
   switch (in.frame) {
     case GX_FRAME_NONE:
       break;
@@ -280,6 +286,7 @@ int gx_potential_create(const gx_component* comps, int32_t ncomp,
   for (int i = 0; i < ncomp; ++i)
     if (comps[i].frame != GX_FRAME_NONE || comps[i].kind == GX_MONARI_BAR)
       h->basic = false;
+  h->kind0 = ncomp > 0 ? comps[0].kind : GX_NULL;
   h->sig = kSigNone;
   if (h->basic && ncomp == 4 && comps[0].kind == GX_MIYAMOTO_NAGAI &&
       comps[1].kind == GX_NFW && comps[2].kind == GX_HERNQUIST &&

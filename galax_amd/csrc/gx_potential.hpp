@@ -342,6 +342,81 @@ struct EvalTable {
 using EvalMilkyWay =
     EvalFixed<GX_MIYAMOTO_NAGAI, GX_NFW, GX_HERNQUIST, GX_HERNQUIST>;
 
+// ------------------------------------- adjoint pieces (origin-centred)
+// For the discrete adjoint: at relative position x and for a cotangent w of
+// the acceleration a(x), add H w (H = Hessian of Phi, so da/dx = -H) to Hw
+// and add (da/dtheta_k) . w to th[k] for the component's table parameters
+// theta = (p0, p1, p2) as given in gx_component (G m, then the lengths).
+// The raw table parameters sit in DComp::p[5..7] (see derive()).
+template <int K>
+__device__ __forceinline__ void comp_adjoint(const DComp& c, double x,
+                                             double y, double z,
+                                             const double (&w)[3],
+                                             double (&Hw)[3], double* th) {
+  const double GM = c.p[5];
+  if constexpr (K == GX_MIYAMOTO_NAGAI) {
+    const double a = c.p[6], b = c.p[7];
+    const double S2 = z * z + b * b, S = sqrt(S2);
+    const double A = a + S;
+    const double D2 = x * x + y * y + A * A, D = sqrt(D2);
+    const double iD3 = 1.0 / (D2 * D), iD5 = iD3 / D2;
+    const double iS = 1.0 / S;
+    const double Az = A * z * iS;
+    const double hxx = GM * (iD3 - 3.0 * x * x * iD5);
+    const double hyy = GM * (iD3 - 3.0 * y * y * iD5);
+    const double hxy = -3.0 * GM * x * y * iD5;
+    const double hxz = -3.0 * GM * x * Az * iD5;
+    const double hyz = -3.0 * GM * y * Az * iD5;
+    const double hzz = GM * (z * z * iD3 * iS * iS + A * b * b * iD3 * iS * iS * iS -
+                             3.0 * Az * Az * iD5);
+    Hw[0] += hxx * w[0] + hxy * w[1] + hxz * w[2];
+    Hw[1] += hxy * w[0] + hyy * w[1] + hyz * w[2];
+    Hw[2] += hxz * w[0] + hyz * w[1] + hzz * w[2];
+    // da/dGM = a/GM
+    th[0] -= (x * w[0] + y * w[1] + Az * w[2]) * iD3;
+    // da/da_param
+    th[1] -= GM * (-3.0 * A * iD5 * (x * w[0] + y * w[1]) +
+                   (z * iS * iD3 - 3.0 * z * A * A * iS * iD5) * w[2]);
+    // da/db
+    th[2] -= GM * (-3.0 * A * b * iS * iD5 * (x * w[0] + y * w[1]) +
+                   z * (b * iS * iS * iD3 - A * b * iS * iS * iS * iD3 -
+                        3.0 * A * A * b * iS * iS * iD5) *
+                       w[2]);
+  } else {
+    const double r2 = x * x + y * y + z * z + kTiny;
+    const double r = sqrt(r2), ir = 1.0 / r;
+    double d1, d2, drs;  // Phi', Phi'', d(Phi'/r)/d r_s
+    if constexpr (K == GX_KEPLER) {
+      d1 = GM * ir * ir;
+      d2 = -2.0 * GM * ir * ir * ir;
+      drs = 0.0;
+    } else if constexpr (K == GX_HERNQUIST) {
+      const double rs = c.p[6], u = 1.0 / (r + rs);
+      d1 = GM * u * u;
+      d2 = -2.0 * GM * u * u * u;
+      drs = -2.0 * GM * u * u * u * ir;
+    } else if constexpr (K == GX_NFW) {
+      const double rs = c.p[6], u = 1.0 / (r + rs), L = log1p(r / rs);
+      d1 = GM * (L * ir * ir - u * ir);
+      d2 = GM * (u * ir * ir - 2.0 * L * ir * ir * ir + (2.0 * r + rs) * u * u * ir * ir);
+      drs = -GM * u * u * ir / rs;
+    } else {  // Plummer
+      const double rs = c.p[6], d = r2 + rs * rs, id = 1.0 / sqrt(d);
+      d1 = GM * r * id * id * id;
+      d2 = GM * (rs * rs - 2.0 * r2) * id * id * id * id * id;
+      drs = -3.0 * GM * rs * id * id * id * id * id;
+    }
+    const double g = d1 * ir;  // a = -g x
+    const double xw = x * w[0] + y * w[1] + z * w[2];
+    const double k = (d2 - g) * ir * ir;
+    Hw[0] += g * w[0] + k * xw * x;
+    Hw[1] += g * w[1] + k * xw * y;
+    Hw[2] += g * w[2] + k * xw * z;
+    th[0] -= (g / GM) * xw;
+    if constexpr (K != GX_KEPLER) th[1] -= drs * xw;
+  }
+}
+
 // ------------------------------------------------ potential and Hessian
 // (not on the integration hot path: each component in the inertial frame)
 __device__ inline double phi_model(const DComp& c, double x, double y,

@@ -195,6 +195,31 @@ int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
                  double* out, int32_t* status, int64_t* nsteps,
                  int64_t* naccept, void* workspace, void* stream);
 
+/* ------------------------------------------------------------- adjoint */
+/* Discrete adjoint of fixed-step (GX_CONSTANT) Dopri8/Dopri5 integration of
+ * n independent orbits, as gx_integrate would run them with the same
+ * arguments.  Given cotangents cot[n*T*6] of the saved states, returns the
+ * cotangents of the initial states w0_bar[n*6] and, per orbit, of the
+ * potential's table parameters theta_bar[n*ntheta] laid out as
+ * (p0, p1, p2) of each component in table order (p0 = G m; p1, p2 the
+ * component's lengths, e.g. Miyamoto-Nagai a, b; Hernquist/NFW/Plummer
+ * r_s).  Supported potentials: the Milky Way composition (Miyamoto-Nagai,
+ * NFW, Hernquist, Hernquist, untranslated) and any single untranslated
+ * Kepler/Hernquist/Miyamoto-Nagai/NFW/Plummer component; ntheta <= 12.
+ * State is checkpointed every K steps; max_steps bounds the steps of any
+ * orbit (it sizes the workspace, gx_adjoint_workspace_bytes(n, max_steps,
+ * K) bytes, 256-byte aligned).  status[n] (optional) is GX_SUCCESS or
+ * GX_MAX_STEPS_REACHED.  Replaces differentiating evaluate_orbit /
+ * MockStreamGenerator.run with jax (dynamics/_src/orbit/solver.py:133,
+ * tests/functional/test_mockstreamgenerator.py:24-116). */
+int64_t gx_adjoint_workspace_bytes(int64_t n, int64_t max_steps, int32_t K);
+int gx_adjoint(const gx_pot* pot, const gx_solver_cfg* cfg, const double* w0,
+               const double* t0, const double* t1, int32_t t_stride,
+               const double* ts, int32_t T, int32_t ts_mode, int64_t n,
+               const double* cot, double* w0_bar, double* theta_bar,
+               int32_t ntheta, int64_t max_steps, int32_t K, int32_t* status,
+               void* workspace, void* stream);
+
 /* Launch-shape override used by the autotuner: workgroup size (multiple of
  * 64, <= 1024) and workgroups per CU for the persistent integrator; 0 keeps
  * the built-in default.  Returns the previous values through the pointers
