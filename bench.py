@@ -60,12 +60,17 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument(
         "--workload",
-        choices=("uneven", "uniform", "longest"),
+        choices=("uneven", "uniform", "longest", "copy", "surrogate"),
         default="uneven",
         help="uneven: the benchmark configuration; uniform: the same orbits with every "
         "integration time set to 200 Myr (balanced work: chip throughput); longest: the "
-        "single most expensive orbit of the configuration alone (per-step latency)",
+        "single most expensive orbit of the configuration alone (per-step latency); copy: a "
+        "plain 1 GiB device-to-device copy (calibrates HBM byte counters, no JSON line); surrogate: "
+        "train the BF16 MLP surrogate of a barred Milky Way with 64 co-rotating subhaloes on 262,144 "
+        "exact orbits, then integrate 262,144 new orbits over 1 Gyr with it and with the exact FP64 field "
+        "(prints its own report line)",
     )
+    ap.add_argument("--dump-steps", default=None, help="save per-orbit attempted steps and queue order (.npz)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -83,6 +88,23 @@ def main() -> int:
     from galax_amd import _lib, flops, workloads
 
     torch.cuda.set_device(0)
+    if args.workload == "copy":
+        nbytes = 1 << 30
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(max(args.steps, 1)):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copied {nbytes} bytes {max(args.steps, 1)} times (read {nbytes}, write {nbytes} each)",
+              flush=True)
+        return 0
+    if args.workload == "surrogate":
+        from galax_amd import surrogate
+
+        rep = surrogate.barred_halo_experiment(log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps({"report": "learned-surrogate field vs exact FP64 (barred MW + 64 subhaloes)", **rep}),
+              flush=True)
+        return 0
     pot = gp.MilkyWayPotential()
     n = args.n
     w0, t0, T = workloads.config_uneven(n=n, seed=2)
@@ -166,6 +188,8 @@ def main() -> int:
             "flop_per_save": flops.f_save("dopri8"),
         },
     }
+    if args.dump_steps:
+        np.savez_compressed(args.dump_steps, nsteps=nsteps, order=order, T=T)
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         out_np = out.cpu().numpy()

@@ -105,3 +105,9 @@ def component_array(comps) -> ctypes.Array:
     for i, c in enumerate(comps):
         arr[i] = c
     return arr
+
+# surrogate GEMM epilogues (gx_mlp_gemm)
+GX_EPI_STORE = 0
+GX_EPI_BIAS = 1
+GX_EPI_BIAS_SILU = 2
+GX_EPI_DSILU = 3

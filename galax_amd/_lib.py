@@ -18,47 +18,59 @@ _lib = None
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
+_f32 = ctypes.c_float
+_f64 = ctypes.c_double
 
 
 class GalaxHipError(RuntimeError):
     """Error returned by the HIP library."""
 
 
+SIGNATURES = {
+    "gx_potential_create": (ctypes.c_int, [_vp, _i32, _vp, _i32, ctypes.POINTER(_vp)]),
+    "gx_potential_destroy": (ctypes.c_int, [_vp]),
+    "gx_acceleration": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _vp]),
+    "gx_gradient": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _vp]),
+    "gx_potential_eval": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _vp]),
+    "gx_hessian": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _vp]),
+    "gx_integrate_workspace_bytes": (_i64, [_i64]),
+    "gx_integrate": (
+        ctypes.c_int,
+        [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    ),
+    "gx_set_launch_shape": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "gx_device_info": (ctypes.c_int, [ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "gx_adjoint_workspace_bytes": (_i64, [_i64, _i64, _i32]),
+    "gx_adjoint": (
+        ctypes.c_int,
+        [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _i64, _vp, _vp, _vp, _i32, _i64, _i32, _vp, _vp, _vp],
+    ),
+    # surrogate kernels
+    "gx_mlp_gemm": (
+        ctypes.c_int,
+        [_vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _i32, _i32, _i32, _vp, _vp, _i64, _vp, _i64,
+         _vp, _i64, _vp],
+    ),
+    "gx_mlp_colsum": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _vp]),
+    "gx_mlp_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _i64, _vp]),
+    "gx_mlp_huber": (ctypes.c_int, [_vp, _i64, _vp, _i64, _f32, _f32, _vp, _i64, _vp, _vp]),
+    "gx_sur_inputs": (ctypes.c_int, [_vp, _vp, _i32, _f64, _i64, _f64, _f64, _vp, _i64, _vp]),
+    "gx_sur_targets": (ctypes.c_int, [_vp, _i32, _i64, _vp, _vp, _f64, _f64, _vp, _vp]),
+    "gx_sur_combine": (
+        ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _f64, _i64, _vp, _i64, _f64, _f64, _f64, _vp, _vp, _vp],
+    ),
+    "gx_sur_check": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _f64, _vp, _vp, _vp]),
+    "gx_method_nodes": (ctypes.c_int, [_i32, _vp, _i32]),
+    "gx_lockstep_stage": (ctypes.c_int, [_i32, _vp, _vp, _vp, _i64, _f64, _vp, _vp]),
+    "gx_lockstep_update": (ctypes.c_int, [_vp, _vp, _vp, _i64, _f64, _vp, _vp]),
+}
+
+
 def _declare(lib: ctypes.CDLL) -> None:
-    sig = {
-        "gx_potential_create": (ctypes.c_int, [_vp, _i32, _vp, _i32, ctypes.POINTER(_vp)]),
-        "gx_potential_destroy": (ctypes.c_int, [_vp]),
-        "gx_acceleration": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _vp]),
-        "gx_gradient": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _vp]),
-        "gx_potential_eval": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _vp]),
-        "gx_hessian": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _vp]),
-        "gx_integrate_workspace_bytes": (_i64, [_i64]),
-        "gx_integrate": (
-            ctypes.c_int,
-            [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
-        ),
-        "gx_set_launch_shape": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
-        "gx_device_info": (ctypes.c_int, [ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
-    }
-    optional = {
-        "gx_adjoint_workspace_bytes": (_i64, [_i64, _i64, _i32]),
-        "gx_adjoint": (
-            ctypes.c_int,
-            [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _i64, _vp, _vp, _vp, _i32, _i64, _i32, _vp, _vp, _vp],
-        ),
-        "gx_mlp_forward": (ctypes.c_int, [_vp, _vp, _i64, _vp, _vp, _vp]),
-        "gx_mlp_train_step": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
-        "gx_mlp_describe": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
-    }
-    for name, (res, args) in sig.items():
+    for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    for name, (res, args) in optional.items():
-        if hasattr(lib, name):
-            fn = getattr(lib, name)
-            fn.restype = res
-            fn.argtypes = args
 
 
 def lib() -> ctypes.CDLL:

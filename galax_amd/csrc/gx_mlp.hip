@@ -1,6 +1,6 @@
 // gx_mlp.hip -- BF16 MFMA kernels for the learned acceleration surrogate.
 //
-// The surrogate is an MLP (5 inputs: x, y, z, cos wt, sin wt; four hidden
+// The surrogate is an MLP (5 inputs: rotating-frame x, y, z, R, r; four hidden
 // layers of 256 with SiLU; 3 outputs) standing in for the expensive part of a
 // composite field (a rotating bar plus dozens of co-rotating subhaloes).
 // Weights and activations are BF16; every product accumulates in FP32 on the
@@ -36,7 +36,12 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 constexpr int BM = 64, BN = 64, BK = 32;
 constexpr int LDSK = BK + 8;  // padded row (bf16 elements) against bank conflicts
 
-enum : int { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_SILU = 2, EPI_DSILU = 3 };
+enum : int {
+  EPI_STORE = GX_EPI_STORE,
+  EPI_BIAS = GX_EPI_BIAS,
+  EPI_BIAS_SILU = GX_EPI_BIAS_SILU,
+  EPI_DSILU = GX_EPI_DSILU
+};
 
 struct GemmArgs {
   const __bf16* A;
@@ -45,7 +50,8 @@ struct GemmArgs {
   long long sbk, sbn;
   long long M, N, K;
   int epi;
-  int accumulate;       // EPI_STORE: add into C
+  int accumulate;       // EPI_STORE: add into C (atomically when split over K)
+  int ksplit;           // blocks along K (grid z); > 1 only with EPI_STORE
   const float* bias;    // [N]
   float* C;             // fp32 output (EPI_STORE / EPI_BIAS); Z for BIAS_SILU
   long long ldc;
@@ -61,13 +67,59 @@ __device__ __forceinline__ float dsilu(float z) {
   return s * (1.0f + z * (1.0f - s));
 }
 
+// Stages a 64 x 32 slab of a strided BF16 operand X(r, k) = X[r*sr + k*sk]
+// (rows r0.., reduction index k0..) into LDS as T[r][k], zero-padding out of
+// range entries.  Three load shapes: 16-byte loads along k (sk == 1), 16-byte
+// loads along r (sr == 1, e.g. the transposed operands of the weight
+// gradient), or scalar loads.
+template <bool VEC>
+__device__ __forceinline__ void stage_tile(__bf16 (*T)[LDSK], const __bf16* X,
+                                           long long sr, long long sk,
+                                           long long r0, long long R,
+                                           long long k0, long long K, int tid) {
+  if (VEC && sk == 1) {
+    const int r = tid >> 2, kc = (tid & 3) * 8;
+    const long long rr = r0 + r, k = k0 + kc;
+    const __bf16* src = X + rr * sr + k;
+    if (rr < R && k + 8 <= K && ((reinterpret_cast<uintptr_t>(src) & 15) == 0)) {
+      *reinterpret_cast<bf16x8*>(&T[r][kc]) = *reinterpret_cast<const bf16x8*>(src);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        T[r][kc + j] = (rr < R && k + j < K) ? X[rr * sr + k + j] : (__bf16)0.0f;
+    }
+  } else if (VEC && sr == 1) {
+    const int kq = tid >> 3, rc = (tid & 7) * 8;
+    const long long k = k0 + kq, rr = r0 + rc;
+    const __bf16* src = X + k * sk + rr;
+    if (k < K && rr + 8 <= R && ((reinterpret_cast<uintptr_t>(src) & 15) == 0)) {
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(src);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) T[rc + j][kq] = v[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        T[rc + j][kq] = (k < K && rr + j < R) ? X[k * sk + rr + j] : (__bf16)0.0f;
+    }
+  } else {
+    const int r = tid >> 2, kc = (tid & 3) * 8;
+    const long long rr = r0 + r;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const long long k = k0 + kc + j;
+      T[r][kc + j] = (rr < R && k < K) ? X[rr * sr + k * sk] : (__bf16)0.0f;
+    }
+  }
+}
+
 template <bool VEC>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   __shared__ __bf16 As[BM][LDSK];
   __shared__ __bf16 Bs[BN][LDSK];  // B transposed: [n][k]
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
+  // m tiles on x (up to 2^31 - 1 blocks), n tiles on y
+  const long long m0 = (long long)blockIdx.x * BM, n0 = (long long)blockIdx.y * BN;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   f32x4 acc[2][2];
 #pragma unroll
@@ -75,34 +127,15 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  for (long long k0 = 0; k0 < g.K; k0 += BK) {
-    // ---- stage A (64 x 32) and B^T (64 x 32): 8 elements per thread each
-    {
-      const int r = tid >> 2, kc = (tid & 3) * 8;
-      const long long m = m0 + r, n = n0 + r;
-      if (VEC && g.sak == 1 && m < g.M && k0 + kc + 8 <= g.K &&
-          ((reinterpret_cast<uintptr_t>(g.A + m * g.sam + k0 + kc) & 15) == 0)) {
-        *reinterpret_cast<bf16x8*>(&As[r][kc]) =
-            *reinterpret_cast<const bf16x8*>(g.A + m * g.sam + k0 + kc);
-      } else {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const long long k = k0 + kc + j;
-          As[r][kc + j] = (m < g.M && k < g.K) ? g.A[m * g.sam + k * g.sak] : (__bf16)0.0f;
-        }
-      }
-      if (VEC && g.sbk == 1 && n < g.N && k0 + kc + 8 <= g.K &&
-          ((reinterpret_cast<uintptr_t>(g.B + n * g.sbn + k0 + kc) & 15) == 0)) {
-        *reinterpret_cast<bf16x8*>(&Bs[r][kc]) =
-            *reinterpret_cast<const bf16x8*>(g.B + n * g.sbn + k0 + kc);
-      } else {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const long long k = k0 + kc + j;
-          Bs[r][kc + j] = (n < g.N && k < g.K) ? g.B[k * g.sbk + n * g.sbn] : (__bf16)0.0f;
-        }
-      }
-    }
+  // this block's share of the reduction (split-K for the weight gradients,
+  // whose reduction runs over the whole batch)
+  const long long kper = ((g.K + g.ksplit - 1) / g.ksplit + BK - 1) / BK * BK;
+  const long long kbeg = (long long)blockIdx.z * kper;
+  const long long kend = kbeg + kper < g.K ? kbeg + kper : g.K;
+  for (long long k0 = kbeg; k0 < kend; k0 += BK) {
+    // ---- stage A (64 x 32) and B^T (64 x 32) into LDS, [row][k] order
+    stage_tile<VEC>(As, g.A, g.sam, g.sak, m0, g.M, k0, kend, tid);
+    stage_tile<VEC>(Bs, g.B, g.sbn, g.sbk, n0, g.N, k0, kend, tid);
     __syncthreads();
     // ---- 2x2 MFMA tiles per wave
     const int kk = (lane >> 4) * 8;
@@ -133,8 +166,12 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
         float v = acc[a][b][j];
         switch (g.epi) {
           case EPI_STORE:
-            if (g.accumulate) v += g.C[m * g.ldc + n];
-            g.C[m * g.ldc + n] = v;
+            if (g.ksplit > 1) {
+              atomicAdd(g.C + m * g.ldc + n, v);
+            } else {
+              if (g.accumulate) v += g.C[m * g.ldc + n];
+              g.C[m * g.ldc + n] = v;
+            }
             break;
           case EPI_BIAS:
             g.C[m * g.ldc + n] = v + g.bias[n];
@@ -168,7 +205,8 @@ __global__ void k_colsum(const __bf16* __restrict__ dz, long long ld, long long 
   atomicAdd(out + n, s);
 }
 
-// Adam on fp32 master parameters; writes the bf16 copy used by the GEMMs.
+// Adam on fp32 master parameters (bias-corrected, as Kingma & Ba 2015);
+// writes the bf16 copy the GEMMs read.
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g,
                        float* __restrict__ m, float* __restrict__ v,
                        __bf16* __restrict__ pb, long long n, float lr, float b1,
@@ -185,49 +223,109 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g,
   if (pb) pb[i] = (__bf16)pi;
 }
 
+// Pseudo-Huber loss on the network's outputs: r = Y - T (first 3 columns),
+// loss = delta^2 (sqrt(1 + (r/delta)^2) - 1) summed, gradient
+// r / sqrt(1 + (r/delta)^2) times `scale` into dY (bf16, other columns 0).
+// The bounded gradient keeps rare large targets (points close to a
+// subhalo's core) from dominating training.
+__global__ void k_huber(const float* __restrict__ Y, long long ldy,
+                        const float* __restrict__ T, long long B, float delta,
+                        float scale, __bf16* __restrict__ dY, long long ldd,
+                        double* __restrict__ loss) {
+  __shared__ double part[256];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double l = 0.0;
+  if (i < B) {
+    const float id = 1.0f / delta;
+    for (int c = 0; c < 3; ++c) {
+      const float r = Y[i * ldy + c] - T[i * 3 + c];
+      const float q = sqrtf(1.0f + (r * id) * (r * id));
+      l += (double)(delta * delta * (q - 1.0f));
+      dY[i * ldd + c] = (__bf16)(scale * r / q);
+    }
+    for (long long c = 3; c < ldd; ++c) dY[i * ldd + c] = (__bf16)0.0f;
+  }
+  part[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, part[0]);
+}
+
 // ------------------------------------------------ surrogate field plumbing
-// Network inputs for points xyz[n*3] at times t (t_stride 0: one time):
-// X[i] = (x/L, y/L, z/L, cos wt, sin wt, 0 ...) as bf16 rows of width ldx.
+// The perturbing part of the field (a bar and subhaloes rotating rigidly at
+// omega) is static in the frame rotating with angle th = omega t, so the
+// network sees rotating-frame positions and predicts the rotating-frame
+// acceleration:  x' = Rz(-th) x,  a = Rz(th) a'.
+
+__device__ __forceinline__ double stage_time(const double* t, int t_stride,
+                                             long long i, double dt) {
+  return t[t_stride ? i : 0] + dt;
+}
+
+// X[i] = (x', y', z, R, r) / L as bf16, zero-padded to width ldx.
 __global__ void k_sur_inputs(const double* __restrict__ xyz,
                              const double* __restrict__ t, int t_stride,
-                             long long n, double inv_len, double omega,
+                             double dt, long long n, double inv_len, double omega,
                              __bf16* __restrict__ X, long long ldx) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double ti = t[t_stride ? i : 0];
   double s, c;
-  sincos(omega * ti, &s, &c);
+  sincos(omega * stage_time(t, t_stride, i, dt), &s, &c);
+  const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  const double R2 = x * x + y * y;
   __bf16* row = X + i * ldx;
-  row[0] = (__bf16)(float)(xyz[3 * i] * inv_len);
-  row[1] = (__bf16)(float)(xyz[3 * i + 1] * inv_len);
-  row[2] = (__bf16)(float)(xyz[3 * i + 2] * inv_len);
-  row[3] = (__bf16)(float)c;
-  row[4] = (__bf16)(float)s;
+  row[0] = (__bf16)(float)((c * x + s * y) * inv_len);
+  row[1] = (__bf16)(float)((c * y - s * x) * inv_len);
+  row[2] = (__bf16)(float)(z * inv_len);
+  row[3] = (__bf16)(float)(sqrt(R2) * inv_len);
+  row[4] = (__bf16)(float)(sqrt(R2 + z * z) * inv_len);
   for (long long k = 5; k < ldx; ++k) row[k] = (__bf16)0.0f;
 }
 
-// a = a_base(x) (exact FP64) + scale * Y[i] (network), or the exact full
-// field where the gate says so: outside the trained ball |x| > r_max, or for
-// orbits flagged by the periodic exact check.
+// Training targets: T[i] = Rz(-th) (a_full - a_base)(x_i, t_i) / scale.
+__global__ void k_sur_targets(const double* __restrict__ t, int t_stride,
+                              long long n, const double* __restrict__ a_full,
+                              const double* __restrict__ a_base, double omega,
+                              double inv_scale, float* __restrict__ T) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s, c;
+  sincos(omega * t[t_stride ? i : 0], &s, &c);
+  const double dx = a_full[3 * i] - a_base[3 * i];
+  const double dy = a_full[3 * i + 1] - a_base[3 * i + 1];
+  const double dz = a_full[3 * i + 2] - a_base[3 * i + 2];
+  T[3 * i] = (float)((c * dx + s * dy) * inv_scale);
+  T[3 * i + 1] = (float)((c * dy - s * dx) * inv_scale);
+  T[3 * i + 2] = (float)(dz * inv_scale);
+}
+
+// The surrogate field: a = a_base(x) (exact FP64) + scale Rz(th) Y[i], or the
+// exact full field where the gate says so (outside the trained ball
+// |x| > r_max, or an orbit flagged by the periodic check).
 __global__ void k_sur_combine(DPot base, DPot full, const double* __restrict__ xyz,
-                              const double* __restrict__ t, int t_stride,
+                              const double* __restrict__ t, int t_stride, double dt,
                               long long n, const float* __restrict__ Y,
-                              long long ldy, double scale, double r_max,
-                              const int* __restrict__ flag,
-                              double* __restrict__ acc, int* __restrict__ used_exact) {
+                              long long ldy, double scale, double omega,
+                              double r_max, const int* __restrict__ flag,
+                              double* __restrict__ acc) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
-  const double ti = t[t_stride ? i : 0];
+  const double ti = stage_time(t, t_stride, i, dt);
   const bool exact = (x * x + y * y + z * z > r_max * r_max) || (flag && flag[i]);
   double ax, ay, az;
   if (exact) {
     accel<true>(full, x, y, z, ti, ax, ay, az);
-    if (used_exact) used_exact[i] = 1;
   } else {
     accel<false>(base, x, y, z, ti, ax, ay, az);
-    ax += scale * (double)Y[i * ldy];
-    ay += scale * (double)Y[i * ldy + 1];
+    double s, c;
+    sincos(omega * ti, &s, &c);
+    const double px = scale * (double)Y[i * ldy], py = scale * (double)Y[i * ldy + 1];
+    ax += c * px - s * py;
+    ay += s * px + c * py;
     az += scale * (double)Y[i * ldy + 2];
   }
   acc[3 * i] = ax;
@@ -235,8 +333,28 @@ __global__ void k_sur_combine(DPot base, DPot full, const double* __restrict__ x
   acc[3 * i + 2] = az;
 }
 
-// Lock-step fixed-step Dopri8 over n orbits with stage accelerations held in
-// HBM: stage positions Q_i, and the step update.  G is [S][n][3].
+// The periodic gate check: the surrogate field at the current positions
+// against the exact field; an orbit whose relative error exceeds `tol` is
+// flagged (sticky) and integrated with the exact field from then on.
+__global__ void k_sur_check(DPot full, const double* __restrict__ xyz,
+                            const double* __restrict__ t, int t_stride, long long n,
+                            const double* __restrict__ a_sur, double tol,
+                            int* __restrict__ flag, float* __restrict__ err) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double ax, ay, az;
+  accel<true>(full, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], t[t_stride ? i : 0], ax, ay,
+              az);
+  const double ex = a_sur[3 * i] - ax, ey = a_sur[3 * i + 1] - ay, ez = a_sur[3 * i + 2] - az;
+  const double e = sqrt((ex * ex + ey * ey + ez * ez) / (ax * ax + ay * ay + az * az));
+  if (err) err[i] = (float)e;
+  if (!(e <= tol)) flag[i] = 1;
+}
+
+// Lock-step fixed-step Dopri8 (Runge-Kutta-Nystrom form, as the persistent
+// integrator) over n orbits sharing one time t[0]; the stage accelerations
+// live in HBM, G = [13][n][3], so any batched field (the surrogate) can
+// produce them.  Stage positions Q_i = q + h (rs_i p + h sum_l (A^2)_il g_l):
 __global__ void k_stage_point(int stage, const double* __restrict__ q,
                               const double* __restrict__ p,
                               const double* __restrict__ G, long long n, double h,
@@ -251,12 +369,17 @@ __global__ void k_stage_point(int stage, const double* __restrict__ q,
     for (int m = 0; m < 3; ++m) s[m] = fma(a, G[((long long)l * n + i) * 3 + m], s[m]);
   }
   const double rs = K.rs[stage];
-  for (int m = 0; m < 3; ++m) Q[3 * i + m] = fma(h, fma(rs, p[3 * i + m], h * s[m]), q[3 * i + m]);
+  for (int m = 0; m < 3; ++m)
+    Q[3 * i + m] = fma(h, fma(rs, p[3 * i + m], h * s[m]), q[3 * i + m]);
 }
 
+// Step update q += h (sb p + h sum bA_l g_l), p += h sum b_l g_l; thread 0
+// also advances the shared time t[0] += h.
 __global__ void k_step_update(double* __restrict__ q, double* __restrict__ p,
-                              const double* __restrict__ G, long long n, double h) {
+                              const double* __restrict__ G, long long n, double h,
+                              double* __restrict__ t) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) t[0] += h;
   if (i >= n) return;
   constexpr auto& K = Dopri8::k;
   double b[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
@@ -276,6 +399,8 @@ __global__ void k_step_update(double* __restrict__ q, double* __restrict__ p,
 
 long long blocks_for(long long n, int bs) { return (n + bs - 1) / bs; }
 
+bool grid_ok(long long n) { return blocks_for(n, 256) <= 0x7fffffffLL; }
+
 }  // namespace
 }  // namespace gx
 
@@ -285,8 +410,8 @@ extern "C" {
 
 int gx_mlp_gemm(const void* A, int64_t sam, int64_t sak, const void* B,
                 int64_t sbk, int64_t sbn, int64_t M, int64_t N, int64_t K,
-                int32_t epi, int32_t accumulate, const float* bias, float* C,
-                int64_t ldc, void* H, int64_t ldh, const float* Zin,
+                int32_t epi, int32_t accumulate, int32_t ksplit, const float* bias,
+                float* C, int64_t ldc, void* H, int64_t ldh, const float* Zin,
                 int64_t ldzin, void* stream) {
   if (M < 0 || N < 0 || K < 0 || !A || !B) return GX_EINVAL;
   if (M == 0 || N == 0) return GX_OK;
@@ -295,6 +420,7 @@ int gx_mlp_gemm(const void* A, int64_t sam, int64_t sak, const void* B,
   if ((epi == EPI_BIAS || epi == EPI_BIAS_SILU) && !bias) return GX_EINVAL;
   if ((epi == EPI_BIAS_SILU || epi == EPI_DSILU) && !H) return GX_EINVAL;
   if (epi == EPI_DSILU && !Zin) return GX_EINVAL;
+  if (ksplit < 1 || ksplit > 65535 || (ksplit > 1 && epi != EPI_STORE)) return GX_EINVAL;
   GemmArgs g{};
   g.A = static_cast<const __bf16*>(A);
   g.sam = sam;
@@ -307,6 +433,7 @@ int gx_mlp_gemm(const void* A, int64_t sam, int64_t sak, const void* B,
   g.K = K;
   g.epi = epi;
   g.accumulate = accumulate;
+  g.ksplit = ksplit;
   g.bias = bias;
   g.C = C;
   g.ldc = ldc;
@@ -314,17 +441,18 @@ int gx_mlp_gemm(const void* A, int64_t sam, int64_t sak, const void* B,
   g.ldh = ldh;
   g.Zin = Zin;
   g.ldzin = ldzin;
-  dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
-  if (grid.y > 65535u * 1024u) return GX_EINVAL;
+  const long long mt = (M + BM - 1) / BM, nt = (N + BN - 1) / BN;
+  if (mt > 0x7fffffffLL || nt > 65535) return GX_EINVAL;
+  dim3 grid((unsigned)mt, (unsigned)nt, (unsigned)ksplit);
   hipLaunchKernelGGL(k_gemm<true>, grid, dim3(256), 0, (hipStream_t)stream, g);
   return check(hipGetLastError());
 }
 
 int gx_mlp_colsum(const void* dz, int64_t ld, int64_t M, int64_t N, float* out,
                   void* stream) {
-  if (!dz || !out || M < 0 || N < 0) return GX_EINVAL;
+  if (!dz || !out || M < 0 || N < 0 || ld < N) return GX_EINVAL;
   if (M == 0 || N == 0) return GX_OK;
-  long long ychunks = M / 4096 + 1;
+  long long ychunks = M / 1024 + 1;
   if (ychunks > 1024) ychunks = 1024;
   dim3 grid((unsigned)blocks_for(N, 256), (unsigned)ychunks);
   hipLaunchKernelGGL(k_colsum, grid, dim3(256), 0, (hipStream_t)stream,
@@ -338,6 +466,7 @@ int gx_mlp_adam(float* p, const float* g, float* m, float* v, void* pb,
                 void* stream) {
   if (!p || !g || !m || !v || n < 0 || step < 1) return GX_EINVAL;
   if (n == 0) return GX_OK;
+  if (!grid_ok(n)) return GX_EINVAL;
   const float c1 = 1.0f / (1.0f - powf(b1, (float)step));
   const float c2 = 1.0f / (1.0f - powf(b2, (float)step));
   hipLaunchKernelGGL(k_adam, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
@@ -346,45 +475,98 @@ int gx_mlp_adam(float* p, const float* g, float* m, float* v, void* pb,
   return check(hipGetLastError());
 }
 
-int gx_sur_inputs(const double* xyz, const double* t, int32_t t_stride, int64_t n,
-                  double inv_len, double omega, void* X, int64_t ldx, void* stream) {
+int gx_mlp_huber(const float* Y, int64_t ldy, const float* T, int64_t B, float delta,
+                 float scale, void* dY, int64_t ldd, double* loss, void* stream) {
+  if (!Y || !T || !dY || !loss || B < 0 || ldy < 3 || ldd < 3 || !(delta > 0.f))
+    return GX_EINVAL;
+  if (B == 0) return GX_OK;
+  if (!grid_ok(B)) return GX_EINVAL;
+  hipLaunchKernelGGL(k_huber, dim3((unsigned)blocks_for(B, 256)), dim3(256), 0,
+                     (hipStream_t)stream, Y, (long long)ldy, T, (long long)B, delta,
+                     scale, static_cast<__bf16*>(dY), (long long)ldd, loss);
+  return check(hipGetLastError());
+}
+
+int gx_sur_inputs(const double* xyz, const double* t, int32_t t_stride, double dt,
+                  int64_t n, double inv_len, double omega, void* X, int64_t ldx,
+                  void* stream) {
   if (!xyz || !t || !X || n < 0 || ldx < 5) return GX_EINVAL;
   if (n == 0) return GX_OK;
+  if (!grid_ok(n)) return GX_EINVAL;
   hipLaunchKernelGGL(k_sur_inputs, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
-                     (hipStream_t)stream, xyz, t, t_stride, (long long)n, inv_len,
+                     (hipStream_t)stream, xyz, t, t_stride, dt, (long long)n, inv_len,
                      omega, static_cast<__bf16*>(X), (long long)ldx);
   return check(hipGetLastError());
 }
 
-int gx_sur_combine(const gx_pot* base, const gx_pot* full, const double* xyz,
-                   const double* t, int32_t t_stride, int64_t n, const float* Y,
-                   int64_t ldy, double scale, double r_max, const int32_t* flag,
-                   double* acc, int32_t* used_exact, void* stream) {
-  if (!base || !full || !xyz || !t || !Y || !acc || n < 0) return GX_EINVAL;
-  if (!base->basic) return GX_EINVAL;
+int gx_sur_targets(const double* t, int32_t t_stride, int64_t n, const double* a_full,
+                   const double* a_base, double omega, double inv_scale, float* T,
+                   void* stream) {
+  if (!t || !a_full || !a_base || !T || n < 0) return GX_EINVAL;
   if (n == 0) return GX_OK;
-  hipLaunchKernelGGL(k_sur_combine, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
-                     (hipStream_t)stream, base->dev, full->dev, xyz, t, t_stride,
-                     (long long)n, Y, (long long)ldy, scale, r_max, flag, acc,
-                     used_exact);
+  if (!grid_ok(n)) return GX_EINVAL;
+  hipLaunchKernelGGL(k_sur_targets, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
+                     (hipStream_t)stream, t, t_stride, (long long)n, a_full, a_base, omega,
+                     inv_scale, T);
   return check(hipGetLastError());
+}
+
+int gx_sur_combine(const gx_pot* base, const gx_pot* full, const double* xyz,
+                   const double* t, int32_t t_stride, double dt, int64_t n,
+                   const float* Y, int64_t ldy, double scale, double omega,
+                   double r_max, const int32_t* flag, double* acc, void* stream) {
+  if (!base || !full || !xyz || !t || !Y || !acc || n < 0 || ldy < 3) return GX_EINVAL;
+  if (!base->basic || base->device != full->device) return GX_EINVAL;
+  if (n == 0) return GX_OK;
+  if (!grid_ok(n)) return GX_EINVAL;
+  hipLaunchKernelGGL(k_sur_combine, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
+                     (hipStream_t)stream, base->dev, full->dev, xyz, t, t_stride, dt,
+                     (long long)n, Y, (long long)ldy, scale, omega, r_max, flag, acc);
+  return check(hipGetLastError());
+}
+
+int gx_sur_check(const gx_pot* full, const double* xyz, const double* t,
+                 int32_t t_stride, int64_t n, const double* a_sur, double tol,
+                 int32_t* flag, float* err, void* stream) {
+  if (!full || !xyz || !t || !a_sur || !flag || n < 0) return GX_EINVAL;
+  if (n == 0) return GX_OK;
+  if (!grid_ok(n)) return GX_EINVAL;
+  hipLaunchKernelGGL(k_sur_check, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
+                     (hipStream_t)stream, full->dev, xyz, t, t_stride, (long long)n, a_sur,
+                     tol, flag, err);
+  return check(hipGetLastError());
+}
+
+int gx_method_nodes(int32_t method, double* c, int32_t cap) {
+  if (method == GX_DOPRI8) {
+    if (c)
+      for (int i = 0; i < 13 && i < cap; ++i) c[i] = Dopri8::k.c[i];
+    return 13;
+  }
+  if (method == GX_DOPRI5) {
+    if (c)
+      for (int i = 0; i < 7 && i < cap; ++i) c[i] = Dopri5::k.c[i];
+    return 7;
+  }
+  return GX_EINVAL;
 }
 
 int gx_lockstep_stage(int32_t stage, const double* q, const double* p,
                       const double* G, int64_t n, double h, double* Q, void* stream) {
   if (stage < 0 || stage >= 13 || !q || !p || !G || !Q || n < 0) return GX_EINVAL;
   if (n == 0) return GX_OK;
+  if (!grid_ok(n)) return GX_EINVAL;
   hipLaunchKernelGGL(k_stage_point, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
                      (hipStream_t)stream, (int)stage, q, p, G, (long long)n, h, Q);
   return check(hipGetLastError());
 }
 
 int gx_lockstep_update(double* q, double* p, const double* G, int64_t n, double h,
-                       void* stream) {
-  if (!q || !p || !G || n < 0) return GX_EINVAL;
-  if (n == 0) return GX_OK;
-  hipLaunchKernelGGL(k_step_update, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
-                     (hipStream_t)stream, q, p, G, (long long)n, h);
+                       double* t, void* stream) {
+  if (!q || !p || !G || !t || n < 0) return GX_EINVAL;
+  if (!grid_ok(n)) return GX_EINVAL;
+  hipLaunchKernelGGL(k_step_update, dim3((unsigned)(n > 0 ? blocks_for(n, 256) : 1)),
+                     dim3(256), 0, (hipStream_t)stream, q, p, G, (long long)n, h, t);
   return check(hipGetLastError());
 }
 

@@ -90,3 +90,44 @@ def cost_order(w0: np.ndarray, T: np.ndarray, mw=None) -> np.ndarray:
     vc = np.sqrt(r * np.abs(np.sum(g * xyz, axis=-1) / np.maximum(r, 1e-12)))
     cost = T * vc / np.maximum(r, 1e-3)
     return np.argsort(-cost, kind="stable").astype(np.int64)
+
+
+# ------------------------------------------------ the surrogate configuration
+BAR_OMEGA_KMS_KPC = 40.0
+
+
+def barred_mw_with_subhaloes(seed: int = 4, n_sub: int = 64):
+    """Milky Way + Monari bar (alpha 0.01, Omega 40 km/s/kpc, R0 8 kpc, v0 220
+    km/s, Rb 3.5 kpc, phi_b 25 deg) + n_sub Plummer subhaloes with
+    m = 10^U(6, 9) Msun, r_s ~ U(0.1, 1) kpc, on circular orbits of radius
+    U(5, 50) kpc in the disk plane co-rotating with the bar.  Returns
+    (full potential, Milky Way part, Omega in rad/Myr)."""
+    from .potential import CompositePotential, MilkyWayPotential, MonariEtAl2016BarPotential, PlummerPotential
+    from .potential import RotatingTranslation, TranslatedPotential
+    from .units import Q
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    omega = BAR_OMEGA_KMS_KPC * 1.022712165045695e-3
+    mw = MilkyWayPotential()
+    bar = MonariEtAl2016BarPotential(
+        alpha=0.01, R0=8.0, v0=Q(220.0, "km/s"), Rb=3.5, phi_b=Q(25.0, "deg"), Omega=omega
+    )
+    m = 10.0 ** rng.uniform(6.0, 9.0, n_sub)
+    rs = rng.uniform(0.1, 1.0, n_sub)
+    R = rng.uniform(5.0, 50.0, n_sub)
+    ph = rng.uniform(0.0, 2.0 * np.pi, n_sub)
+    subs = {
+        f"sub{i}": TranslatedPotential(
+            PlummerPotential(m_tot=m[i], r_s=rs[i]),
+            RotatingTranslation([R[i] * np.cos(ph[i]), R[i] * np.sin(ph[i]), 0.0], omega),
+        )
+        for i in range(n_sub)
+    }
+    full = CompositePotential(mw=mw, bar=bar, **subs)
+    return full, mw, omega
+
+
+def surrogate_orbits(n: int, seed: int) -> np.ndarray:
+    """(n, 6) disk orbits for the surrogate configuration: R ~ U(3, 30) kpc,
+    10% dispersion (seed 4 trains, seed 5 benchmarks)."""
+    return disk_ics(n, seed, R_range=(3.0, 30.0), z_sigma=0.3, dispersion=0.1)

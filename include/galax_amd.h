@@ -231,6 +231,80 @@ int gx_set_launch_shape(int32_t block, int32_t blocks_per_cu,
  * the current device, and the VGPR count of the integrator kernels. */
 int gx_device_info(int32_t* num_cu, int32_t* warp_size);
 
+/* ------------------------------------------------------------- surrogate */
+/* The learned acceleration surrogate has no counterpart in the reference:
+ * galax evaluates every field exactly (potential/_src/base.py:170-179).
+ * These entry points let a caller train an MLP on the perturbing part of a
+ * field (here a bar plus co-rotating subhaloes) and integrate with it.
+ * Matrices are row-major BF16 (uint16 storage) unless stated; all pointers
+ * are device pointers; every call is asynchronous on `stream`.
+ *
+ * gx_mlp_gemm: C(m, n) = sum_k A(m, k) B(k, n) on the BF16 matrix cores with
+ * FP32 accumulation, where A(m, k) = A[m*sam + k*sak] and B(k, n) =
+ * B[k*sbk + n*sbn].  epi selects the epilogue:
+ *   GX_EPI_STORE      C[m*ldc + n] = acc (+= if accumulate; with ksplit > 1
+ *                     the K range is split over ksplit workgroups that add
+ *                     atomically, so C must hold the initial value);
+ *   GX_EPI_BIAS       C[m*ldc + n] = acc + bias[n];
+ *   GX_EPI_BIAS_SILU  z = acc + bias[n]; C[m*ldc + n] = z (if C != NULL);
+ *                     H[m*ldh + n] = bf16(silu(z));
+ *   GX_EPI_DSILU      H[m*ldh + n] = bf16(acc * silu'(Zin[m*ldzin + n])). */
+#define GX_EPI_STORE 0
+#define GX_EPI_BIAS 1
+#define GX_EPI_BIAS_SILU 2
+#define GX_EPI_DSILU 3
+int gx_mlp_gemm(const void* A, int64_t sam, int64_t sak, const void* B,
+                int64_t sbk, int64_t sbn, int64_t M, int64_t N, int64_t K,
+                int32_t epi, int32_t accumulate, int32_t ksplit, const float* bias,
+                float* C, int64_t ldc, void* H, int64_t ldh, const float* Zin,
+                int64_t ldzin, void* stream);
+/* out[n] += sum_m dz[m*ld + n] (bias gradients; out must hold the start). */
+int gx_mlp_colsum(const void* dz, int64_t ld, int64_t M, int64_t N, float* out,
+                  void* stream);
+/* One bias-corrected Adam step (step >= 1) on FP32 parameters p with
+ * gradient g and moments m, v; pb (optional) receives the BF16 copy. */
+int gx_mlp_adam(float* p, const float* g, float* m, float* v, void* pb,
+                int64_t n, float lr, float b1, float b2, float eps, int64_t step,
+                void* stream);
+/* Pseudo-Huber loss of Y[:, 0:3] (row stride ldy) against T[B][3]:
+ * *loss += sum delta^2 (sqrt(1 + (r/delta)^2) - 1) (loss must hold the
+ * start), dY[:, 0:3] = bf16(scale * r / sqrt(1 + (r/delta)^2)), other
+ * columns of dY (row stride ldd) zero. */
+int gx_mlp_huber(const float* Y, int64_t ldy, const float* T, int64_t B, float delta,
+                 float scale, void* dY, int64_t ldd, double* loss, void* stream);
+/* Network inputs at points xyz[n][3] and times t[0] + dt (t_stride 0) or
+ * t[i] + dt (t_stride 1): X[i] = bf16((x', y', z, R, r) / L, 0...) with
+ * (x', y') the position rotated by -omega t, rows of width ldx >= 5. */
+int gx_sur_inputs(const double* xyz, const double* t, int32_t t_stride, double dt,
+                  int64_t n, double inv_len, double omega, void* X, int64_t ldx,
+                  void* stream);
+/* Training targets T[i] = float(Rz(-omega t_i) (a_full - a_base)_i / scale). */
+int gx_sur_targets(const double* t, int32_t t_stride, int64_t n, const double* a_full,
+                   const double* a_base, double omega, double inv_scale, float* T,
+                   void* stream);
+/* Surrogate field: acc[i] = a_base(x_i) + scale Rz(omega t) Y[i][0:3], or
+ * the exact field of `full` where |x_i| > r_max or flag[i] != 0 (flag may be
+ * NULL).  `base` must be an untranslated table without bars. */
+int gx_sur_combine(const gx_pot* base, const gx_pot* full, const double* xyz,
+                   const double* t, int32_t t_stride, double dt, int64_t n,
+                   const float* Y, int64_t ldy, double scale, double omega,
+                   double r_max, const int32_t* flag, double* acc, void* stream);
+/* Gate check: flag[i] = 1 (sticky) where |a_sur - a_full| / |a_full| > tol
+ * at xyz[i]; err[i] (optional) receives the relative error. */
+int gx_sur_check(const gx_pot* full, const double* xyz, const double* t,
+                 int32_t t_stride, int64_t n, const double* a_sur, double tol,
+                 int32_t* flag, float* err, void* stream);
+/* Stage nodes c_i of a method (time of stage i = t + c_i h): writes up to
+ * cap of them to c (host memory, may be NULL) and returns the stage count. */
+int gx_method_nodes(int32_t method, double* c, int32_t cap);
+/* Lock-step fixed-step Dopri8 for batched fields: stage positions
+ * Q = q + h (rs_stage p + h sum_l (A^2)_{stage,l} G[l]) from the stage
+ * accelerations G[13][n][3]; and the step update of q, p (and t[0] += h). */
+int gx_lockstep_stage(int32_t stage, const double* q, const double* p,
+                      const double* G, int64_t n, double h, double* Q, void* stream);
+int gx_lockstep_update(double* q, double* p, const double* G, int64_t n, double h,
+                       double* t, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,20 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_ctypes_signatures_match_header():
+    """Every declared entry point has a ctypes signature with the same number
+    of arguments (a mismatch would pass garbage through the ABI)."""
+    text = HEADER.read_text()
+    protos = dict(
+        (m.group(1), m.group(2))
+        for m in re.finditer(r"^(?:int|int64_t)\s+(gx_\w+)\s*\(([^)]*)\)\s*;", text, flags=re.M | re.S)
+    )
+    assert set(protos) == set(_lib.SIGNATURES), set(protos) ^ set(_lib.SIGNATURES)
+    for name, args in protos.items():
+        nargs = 0 if args.strip() in ("", "void") else args.count(",") + 1
+        assert nargs == len(_lib.SIGNATURES[name][1]), name
+
+
 def test_struct_layouts_match_header(tmp_path):
     src = tmp_path / "layout.c"
     src.write_text(
