@@ -49,6 +49,63 @@ def cpu_baseline(pot, solver, w0, T, nsave, out_gpu, seconds: float, threads: in
     return steps / elapsed, done, elapsed, worst
 
 
+def streamfit_report(steps: int) -> dict:
+    """Forward + backward of the stream-track loss, timed over `steps`
+    repetitions, with the gradient checked against central differences
+    (h = 1e-6 relative) of the GPU forward pass."""
+    import numpy as np
+    import torch
+
+    from galax_amd.adjoint import MilkyWayParams
+    from galax_amd.streamfit import make_problem, stream_loss
+
+    torch.cuda.set_device(0)
+    prob = make_problem(n_release=10_000, span=3000.0, seed=6, n_target=200, dt=1.0, K=64)
+    theta0 = MilkyWayParams.default.copy()
+
+    def fwd_bwd():
+        th = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+        L = stream_loss(th, prob)
+        L.backward()
+        return float(L), th.grad.numpy().copy()
+
+    fwd_bwd()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(max(steps, 1)):
+        L, g = fwd_bwd()
+    torch.cuda.synchronize()
+    ms_grad = (time.perf_counter() - t) / max(steps, 1) * 1e3
+    t = time.perf_counter()
+    with torch.no_grad():
+        stream_loss(torch.tensor(theta0), prob)
+    torch.cuda.synchronize()
+    ms_fwd = (time.perf_counter() - t) * 1e3
+    fd = []
+    with torch.no_grad():
+        for j in range(6):
+            h = 1e-6 * theta0[j]
+            tp, tm = theta0.copy(), theta0.copy()
+            tp[j] += h
+            tm[j] -= h
+            fd.append((float(stream_loss(torch.tensor(tp), prob)) - float(stream_loss(torch.tensor(tm), prob)))
+                      / (2 * h))
+    fd = np.array(fd)
+    return {
+        "report": "stream-track loss gradient by discrete adjoint (checkpoint every 64 steps)",
+        "particles": prob.n_particles,
+        "release_span_myr": 3000.0,
+        "dt_myr": 1.0,
+        "loss": L,
+        "grad": dict(zip(MilkyWayParams.names, g.tolist())),
+        "grad_fd": dict(zip(MilkyWayParams.names, fd.tolist())),
+        "max_rel_diff_vs_fd": float(np.max(np.abs(g - fd) / np.abs(fd))),
+        "ms_forward": ms_fwd,
+        "ms_forward_backward": ms_grad,
+        "ms_central_fd_6_params": 12 * ms_fwd,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,7 +117,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument(
         "--workload",
-        choices=("uneven", "uniform", "longest", "copy", "surrogate"),
+        choices=("uneven", "uniform", "longest", "copy", "surrogate", "streamfit"),
         default="uneven",
         help="uneven: the benchmark configuration; uniform: the same orbits with every "
         "integration time set to 200 Myr (balanced work: chip throughput); longest: the "
@@ -68,7 +125,9 @@ def main() -> int:
         "plain 1 GiB device-to-device copy (calibrates HBM byte counters, no JSON line); surrogate: "
         "train the BF16 MLP surrogate of a barred Milky Way with 64 co-rotating subhaloes on 262,144 "
         "exact orbits, then integrate 262,144 new orbits over 1 Gyr with it and with the exact FP64 field "
-        "(prints its own report line)",
+        "(prints its own report line); streamfit: gradient of a stream-track loss (20,000 particles "
+        "released over 3 Gyr, fixed-step Dopri8 h = 1 Myr) w.r.t. six Milky Way parameters by the "
+        "discrete adjoint, timed and checked against central differences (prints its own report line)",
     )
     ap.add_argument("--dump-steps", default=None, help="save per-orbit attempted steps and queue order (.npz)")
     args = ap.parse_args()
@@ -104,6 +163,9 @@ def main() -> int:
         rep = surrogate.barred_halo_experiment(log=lambda m: print(m, file=sys.stderr, flush=True))
         print(json.dumps({"report": "learned-surrogate field vs exact FP64 (barred MW + 64 subhaloes)", **rep}),
               flush=True)
+        return 0
+    if args.workload == "streamfit":
+        print(json.dumps(streamfit_report(args.steps)), flush=True)
         return 0
     pot = gp.MilkyWayPotential()
     n = args.n
