@@ -8,11 +8,15 @@ library or a GPU is missing, calls raise.
 from __future__ import annotations
 
 import ctypes
+import os
 from pathlib import Path
 
 from . import _abi
 
-LIB_PATH = Path(__file__).resolve().parent / "libgalaxhip.so"
+# GALAX_AMD_LIB names another build of the same library (e.g. one compiled
+# with different options, for timing comparisons); the default is the in-tree
+# build next to this file.
+LIB_PATH = Path(os.environ.get("GALAX_AMD_LIB") or Path(__file__).resolve().parent / "libgalaxhip.so")
 _lib = None
 
 _vp = ctypes.c_void_p

@@ -551,7 +551,7 @@ __global__ void k_zero_counter(unsigned long long* c) {
 template <class M, bool PID, class E>
 int launch(const gx_pot* pot, IArgs& A, hipStream_t stream) {
   const auto kern = k_integrate<M, PID, E>;
-  constexpr bool FULL = E::kLarge;
+  constexpr bool FULL = E::kFrames;  // the prepass's generic field must see frames
   const LaunchShape& ls = launch_shape();
   const int block = kBlock;
   int occ = 0;
@@ -654,6 +654,9 @@ int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
     if (pot->sig == kSigMilkyWay)
       return pid ? launch<Dopri8, true, EvalMilkyWay>(pot, A, s)
                  : launch<Dopri8, false, EvalMilkyWay>(pot, A, s);
+    if (pot->sig == kSigMilkyWayTracked)
+      return pid ? launch<Dopri8, true, EvalMilkyWayTracked>(pot, A, s)
+                 : launch<Dopri8, false, EvalMilkyWayTracked>(pot, A, s);
     if (pid)
       return pot->basic ? launch<Dopri8, true, Basic>(pot, A, s)
                         : launch<Dopri8, true, Full>(pot, A, s);

@@ -18,7 +18,10 @@ struct gx_pot {
   int kind0 = 0;      // kind of the first component
 };
 
-enum : int { kSigNone = 0, kSigMilkyWay = 1 };
+// Table signatures with a compile-time evaluator: the Milky Way model, and
+// the Milky Way model plus one Plummer sphere on a track (the mock-stream
+// field with the progenitor's own potential).
+enum : int { kSigNone = 0, kSigMilkyWay = 1, kSigMilkyWayTracked = 2 };
 
 namespace gx {
 

@@ -230,8 +230,7 @@ int gx_potential_create(const gx_component* comps, int32_t ncomp,
           !std::isfinite(tr.v[3 * i]) || !std::isfinite(tr.v[3 * i + 1]) ||
           !std::isfinite(tr.v[3 * i + 2]))
         return GX_EINVAL;
-    knot_bytes += align256(sizeof(double) * tr.n) +
-                  2 * align256(sizeof(double) * 3 * tr.n);
+    knot_bytes += align256(sizeof(double) * 16 * (tr.n > 1 ? tr.n - 1 : 1));
   }
   const size_t comp_bytes = align256(sizeof(DComp) * dc.size());
   const size_t track_bytes = align256(sizeof(DTrack) * (ntracks > 0 ? ntracks : 1));
@@ -256,16 +255,34 @@ int gx_potential_create(const gx_component* comps, int32_t ncomp,
     DTrack d{};
     d.n = tr.n;
     d.t0 = tr.t[0];
+    d.t1 = tr.t[tr.n - 1];
     d.inv_h = tr.n > 1 ? (double)(tr.n - 1) / (tr.t[tr.n - 1] - tr.t[0]) : 0.0;
-    std::memcpy(host.data() + off, tr.t, sizeof(double) * tr.n);
-    d.t = reinterpret_cast<const double*>(base + off);
-    off += align256(sizeof(double) * tr.n);
-    std::memcpy(host.data() + off, tr.q, sizeof(double) * 3 * tr.n);
-    d.q = reinterpret_cast<const double*>(base + off);
-    off += align256(sizeof(double) * 3 * tr.n);
-    std::memcpy(host.data() + off, tr.v, sizeof(double) * 3 * tr.n);
-    d.v = reinterpret_cast<const double*>(base + off);
-    off += align256(sizeof(double) * 3 * tr.n);
+    for (int m = 0; m < 3; ++m) {
+      d.q0[m] = tr.q[m];
+      d.q1[m] = tr.q[3 * (tr.n - 1) + m];
+    }
+    // cubic Hermite interpolant of (q, v) at the knots in power form:
+    // c0 = q_k, c1 = h v_k, c2 = 3 (q_{k+1} - q_k) - h (2 v_k + v_{k+1}),
+    // c3 = 2 (q_k - q_{k+1}) + h (v_k + v_{k+1})
+    double* rec = reinterpret_cast<double*>(host.data() + off);
+    for (int64_t i = 0; i + 1 < tr.n; ++i) {
+      double* r = rec + 16 * i;
+      const double hk = tr.t[i + 1] - tr.t[i];
+      r[0] = tr.t[i];
+      r[1] = tr.t[i + 1];
+      r[2] = 1.0 / hk;
+      r[3] = 0.0;
+      for (int m = 0; m < 3; ++m) {
+        const double qa = tr.q[3 * i + m], qb = tr.q[3 * (i + 1) + m];
+        const double va = hk * tr.v[3 * i + m], vb = hk * tr.v[3 * (i + 1) + m];
+        r[4 + 4 * m] = qa;
+        r[5 + 4 * m] = va;
+        r[6 + 4 * m] = 3.0 * (qb - qa) - 2.0 * va - vb;
+        r[7 + 4 * m] = 2.0 * (qa - qb) + va + vb;
+      }
+    }
+    d.rec = reinterpret_cast<const double*>(base + off);
+    off += align256(sizeof(double) * 16 * (tr.n > 1 ? tr.n - 1 : 1));
     dt[k] = d;
   }
   std::memcpy(host.data() + comp_bytes, dt.data(), sizeof(DTrack) * dt.size());
@@ -292,6 +309,12 @@ int gx_potential_create(const gx_component* comps, int32_t ncomp,
       comps[1].kind == GX_NFW && comps[2].kind == GX_HERNQUIST &&
       comps[3].kind == GX_HERNQUIST)
     h->sig = kSigMilkyWay;
+  bool mw_origin = ncomp == 5;
+  for (int i = 0; mw_origin && i < 4; ++i) mw_origin = comps[i].frame == GX_FRAME_NONE;
+  if (mw_origin && comps[0].kind == GX_MIYAMOTO_NAGAI && comps[1].kind == GX_NFW &&
+      comps[2].kind == GX_HERNQUIST && comps[3].kind == GX_HERNQUIST &&
+      comps[4].kind == GX_PLUMMER && comps[4].frame == GX_FRAME_TRACK)
+    h->sig = kSigMilkyWayTracked;
   *out = h;
   return GX_OK;
 }

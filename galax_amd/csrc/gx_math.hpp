@@ -24,12 +24,19 @@ namespace gx {
 
 // 1/d from the hardware reciprocal estimate and two Newton-Raphson steps
 // (each squares the relative error), ending within an ulp.
+#ifndef GX_NEWTON
+#define GX_NEWTON 2
+#endif
+
 __device__ __forceinline__ double rcp(double d) {
   double r = __builtin_amdgcn_rcp(d);
   double e = fma(-d, r, 1.0);
   r = fma(r, e, r);
+#if GX_NEWTON > 1
   e = fma(-d, r, 1.0);
-  return fma(r, e, r);
+  r = fma(r, e, r);
+#endif
+  return r;
 }
 
 // 1/sqrt(x) for finite x > 0 from the hardware estimate and two Newton
@@ -39,9 +46,12 @@ __device__ __forceinline__ double rsqrt_fast(double x) {
   double h = 0.5 * r;
   double e = fma(-x * r, h, 0.5);
   r = fma(r, e, r);
+#if GX_NEWTON > 1
   h = 0.5 * r;
   e = fma(-x * r, h, 0.5);
-  return fma(r, e, r);
+  r = fma(r, e, r);
+#endif
+  return r;
 }
 
 // n/d: Newton reciprocal, then one residual correction of the quotient.
@@ -51,7 +61,15 @@ __device__ __forceinline__ double div(double n, double d) {
   return fma(fma(-d, q, n), r, q);
 }
 
+// log1p(x) given inv_u ~ 1/(1 + x) to within a few ulp (the caller often
+// has it already); used only for the rounding correction of 1 + x.
+__device__ __forceinline__ double log1p_inv(double x, double inv_u);
+
 __device__ __forceinline__ double log1p_fast(double x) {
+  return log1p_inv(x, rcp(1.0 + x));
+}
+
+__device__ __forceinline__ double log1p_inv(double x, double inv_u) {
   constexpr double kLn2Hi = 0x1.62e42fef00000p-1;
   constexpr double kLn2Lo = 0x1.473de6af278edp-34;
   constexpr double kSqrtHalf = 0.70710678118654752440;
@@ -81,7 +99,7 @@ __device__ __forceinline__ double log1p_fast(double x) {
   const double hfsq = 0.5 * f * f;
   const double dk = (double)k;
   // e/u: the whole result when x is below an ulp of 1, so keep it accurate
-  const double corr = e * rcp(u);
+  const double corr = e * inv_u;
   return dk * kLn2Hi - ((hfsq - (s * (hfsq + R) + (dk * kLn2Lo + corr))) - f);
 }
 

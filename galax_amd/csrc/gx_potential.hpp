@@ -52,13 +52,18 @@ struct DComp {
   double f[6];
 };
 
+// A track as one 128-byte record per knot interval k (t_k <= t < t_{k+1}):
+//   rec[16k + 0, 1, 2] = t_k, t_{k+1}, 1 / (t_{k+1} - t_k)
+//   rec[16k + 4 + 4m .. 7 + 4m] = power-basis coefficients c0..c3 of
+//   coordinate m of the cubic Hermite interpolant in s = (t - t_k)/(t_{k+1} - t_k)
+// so that a lookup touches one cache line and its loads are independent.
 struct DTrack {
-  const double* t;
-  const double* q;
-  const double* v;
-  long long n;
-  double t0;
-  double inv_h;  // 1/(mean knot spacing); used to guess the interval
+  const double* rec;
+  long long n;      // knots (records: n - 1)
+  double t0, t1;    // first and last knot time
+  double inv_h;     // (n - 1) / (t1 - t0): guesses the interval
+  double q0[3];     // position held before the first knot
+  double q1[3];     // and after the last
 };
 
 struct DPot {
@@ -71,37 +76,35 @@ struct DPot {
 // ------------------------------------------------------------- tracks
 __device__ inline void track_pos(const DTrack& T, double t, double& x,
                                  double& y, double& z) {
-  const long long n = T.n;
-  long long k;
-  if (n <= 1 || t <= T.t[0]) {
-    x = T.q[0];
-    y = T.q[1];
-    z = T.q[2];
+  if (T.n <= 1 || t <= T.t0) {
+    x = T.q0[0];
+    y = T.q0[1];
+    z = T.q0[2];
     return;
   }
-  if (t >= T.t[n - 1]) {
-    x = T.q[3 * (n - 1)];
-    y = T.q[3 * (n - 1) + 1];
-    z = T.q[3 * (n - 1) + 2];
+  if (t >= T.t1) {
+    x = T.q1[0];
+    y = T.q1[1];
+    z = T.q1[2];
     return;
   }
-  // guess from the mean spacing, then correct to the knot interval that
-  // holds t (t_k <= t < t_{k+1}).
-  double u = (t - T.t0) * T.inv_h;
-  k = (long long)floor(u);
-  k = k < 0 ? 0 : (k > n - 2 ? n - 2 : k);
-  while (k > 0 && T.t[k] > t) --k;
-  while (k < n - 2 && T.t[k + 1] <= t) ++k;
-  const double tk = T.t[k], hk = T.t[k + 1] - tk;
-  const double s = (t - tk) / hk;
-  const double s2 = s * s, s3 = s2 * s;
-  const double h00 = 2 * s3 - 3 * s2 + 1, h10 = s3 - 2 * s2 + s;
-  const double h01 = -2 * s3 + 3 * s2, h11 = s3 - s2;
-  const double* q0 = T.q + 3 * k;
-  const double* v0 = T.v + 3 * k;
-  x = h00 * q0[0] + h10 * hk * v0[0] + h01 * q0[3] + h11 * hk * v0[3];
-  y = h00 * q0[1] + h10 * hk * v0[1] + h01 * q0[4] + h11 * hk * v0[4];
-  z = h00 * q0[2] + h10 * hk * v0[2] + h01 * q0[5] + h11 * hk * v0[5];
+  // guess from the mean spacing, then correct to the interval holding t
+  const long long last = T.n - 2;
+  long long k = (long long)floor((t - T.t0) * T.inv_h);
+  k = k < 0 ? 0 : (k > last ? last : k);
+  const double* r = T.rec + 16 * k;
+  while (k > 0 && r[0] > t) {
+    --k;
+    r -= 16;
+  }
+  while (k < last && r[1] <= t) {
+    ++k;
+    r += 16;
+  }
+  const double s = (t - r[0]) * r[2];
+  x = fma(s, fma(s, fma(s, r[7], r[6]), r[5]), r[4]);
+  y = fma(s, fma(s, fma(s, r[11], r[10]), r[9]), r[8]);
+  z = fma(s, fma(s, fma(s, r[15], r[14]), r[13]), r[12]);
 }
 
 __device__ inline void frame_delta(const DPot& P, const DComp& c, double t,
@@ -308,6 +311,7 @@ __device__ __forceinline__ void comp_acc(const DComp& c, double x, double y,
 template <int... K>
 struct EvalFixed {
   static constexpr bool kLarge = false;
+  static constexpr bool kFrames = false;  // origin-centred table only
   template <size_t... I>
   __device__ __forceinline__ static void run(const DPot& P, double x, double y,
                                              double z, double& ax, double& ay,
@@ -331,6 +335,7 @@ struct EvalFixed {
 template <bool FULL>
 struct EvalTable {
   static constexpr bool kLarge = FULL;
+  static constexpr bool kFrames = FULL;
   __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
                                              double z, double t, double& ax,
                                              double& ay, double& az) {
@@ -339,8 +344,75 @@ struct EvalTable {
 };
 
 // galax's MilkyWayPotential: disk, halo, bulge, nucleus (milkyway.py:173-236)
+// The Milky Way model (builtin/milkyway.py:173-236; rows 0-3 = Miyamoto-
+// Nagai disk, NFW halo, Hernquist bulge, Hernquist nucleus) written out for
+// the step loop: the three spherical components share one reciprocal,
+// 1/(d_h d_b d_n), from which each 1/d_k follows by two products, and the
+// halo's log1p takes 1/(1 + r/r_s) = r_s/d_h from it instead of a fourth
+// reciprocal.
+struct EvalMW {
+  static constexpr bool kLarge = false;
+  static constexpr bool kFrames = false;
+  __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
+                                             double z, double /*t*/,
+                                             double& ax, double& ay,
+                                             double& az) {
+    const double* dk = P.comps[0].p;
+    const double* hk = P.comps[1].p;
+    const double* bk = P.comps[2].p;
+    const double* nk = P.comps[3].p;
+    const double r2 = x * x + y * y + z * z;
+    const double ir = rsqrt_fast(r2 + kTiny);
+    const double r = (r2 + kTiny) * ir;
+    // disk
+    const double zb = z * z + dk[2];
+    const double iS = rsqrt_fast(zb);
+    const double S = zb * iS;
+    const double A = dk[1] + S;
+    const double iD = rsqrt_fast(x * x + y * y + A * A);
+    const double gd = dk[0] * iD * iD * iD;
+    // spherical components
+    const double dh = r + hk[1], db = r + bk[1], dn = r + nk[1];
+    const double dbn = db * dn;
+    const double R = rcp(dh * dbn);
+    const double uh = dbn * R, ub = dh * dn * R, un = dh * db * R;
+    const double L = log1p_inv(r * hk[2], hk[1] * uh);
+    const double g = hk[0] * ir * ir * (L * ir - uh) + bk[0] * ub * ub * ir +
+                     nk[0] * un * un * ir;
+    ax = -(gd + g) * x;
+    ay = -(gd + g) * y;
+    az = -(gd * A * iS + g) * z;
+  }
+};
+
+#ifdef GX_MW_SHARED
+using EvalMilkyWay = EvalMW;
+#else
 using EvalMilkyWay =
     EvalFixed<GX_MIYAMOTO_NAGAI, GX_NFW, GX_HERNQUIST, GX_HERNQUIST>;
+#endif
+
+// The mock-stream field: the Milky Way model (rows 0-3) plus a Plummer
+// sphere riding a track (row 4), as MockStreamGenerator builds it with a
+// progenitor potential (legacy/mockstream/mockstream_generator.py:161-275).
+struct EvalMilkyWayTracked {
+  static constexpr bool kLarge = false;
+  static constexpr bool kFrames = true;
+  __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
+                                             double z, double t, double& ax,
+                                             double& ay, double& az) {
+    EvalMilkyWay::acc(P, x, y, z, t, ax, ay, az);
+    const DComp& c = P.comps[4];
+    double dx, dy, dz;
+    track_pos(P.tracks[c.track], t, dx, dy, dz);
+    const double X = x - dx, Y = y - dy, Z = z - dz;
+    const double id = rsqrt_fast(X * X + Y * Y + Z * Z + c.p[1]);
+    const double g = c.p[0] * id * id * id;
+    ax -= g * X;
+    ay -= g * Y;
+    az -= g * Z;
+  }
+};
 
 // ------------------------------------- adjoint pieces (origin-centred)
 // For the discrete adjoint: at relative position x and for a cotangent w of

@@ -111,7 +111,7 @@ class AbstractPotential:
         if is_torch:
             xq = qv.to(device="cuda", dtype=torch.float64)
         else:
-            xq = torch.as_tensor(np.asarray(qv, dtype=np.float64), device="cuda")
+            xq = torch.as_tensor(np.ascontiguousarray(qv, dtype=np.float64), device="cuda")
         if xq.shape[-1:] != (3,):
             raise ValueError(f"positions must have shape (..., 3), got {tuple(xq.shape)}")
         batch = xq.shape[:-1]
@@ -121,7 +121,7 @@ class AbstractPotential:
         if isinstance(tv, torch.Tensor):
             tt = tv.to(device="cuda", dtype=torch.float64)
         else:
-            tt = torch.as_tensor(np.asarray(tv, dtype=np.float64), device="cuda")
+            tt = torch.as_tensor(np.ascontiguousarray(tv, dtype=np.float64), device="cuda")
         if tt.numel() == 1:
             tt = tt.reshape(1).contiguous()
             stride = 0
