@@ -44,6 +44,12 @@ SIGNATURES = {
     ),
     "gx_set_launch_shape": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "gx_device_info": (ctypes.c_int, [ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "gx_math_check": (ctypes.c_int, [_vp, _i64, _vp, _vp]),
+    "gx_nbody_acceleration": (ctypes.c_int, [_vp, _vp, _vp, _f64, _f64, _i64, _vp, _vp]),
+    "gx_nbody_workspace_bytes": (_i64, [_i64]),
+    "gx_nbody_integrate": (
+        ctypes.c_int, [_vp, _vp, _f64, _vp, _vp, _f64, _f64, _vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    ),
     "gx_adjoint_workspace_bytes": (_i64, [_i64, _i64, _i32]),
     "gx_adjoint": (
         ctypes.c_int,

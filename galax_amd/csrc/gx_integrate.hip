@@ -56,6 +56,7 @@ struct IArgs {
 
 constexpr int kBlock = 256;
 constexpr long long kStepCap = 10000000;  // attempted steps per orbit
+constexpr long long kHeadLanes = 4096;     // lanes that start on the queue head
 
 // One evaluation of method M from (tb, q, p) with step h (solver time tau;
 // physical time dir * tau, velocities pre-multiplied by dir).
@@ -341,6 +342,10 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
   const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long orb = ticket_to_orbit(A, lane);
   bool fresh = true;
+  // The first tickets of a cost-ordered queue hold the longest orbits, whose
+  // step latency bounds the launch: their waves issue ahead of the others
+  // on a shared SIMD (wave priority only; the queue is unchanged).
+  if ((long long)blockIdx.x * blockDim.x < kHeadLanes) __builtin_amdgcn_s_setprio(3);
 
   double q[3], p[3];
   double tau = 0.0, tnext = 0.0, dir = 1.0;

@@ -145,6 +145,19 @@ __global__ void k_potential(DPot P, const double* __restrict__ xyz,
   }
 }
 
+// The short FP64 primitives of gx_math.hpp at given arguments (accuracy
+// checks): out[4i..4i+3] = rcp(x), rsqrt_fast(x), log1p_fast(x), div(1, x).
+__global__ void k_math_check(const double* __restrict__ x, long long n,
+                             double* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = x[i];
+  out[4 * i] = rcp(v);
+  out[4 * i + 1] = rsqrt_fast(v);
+  out[4 * i + 2] = log1p_fast(v);
+  out[4 * i + 3] = div(1.0, v);
+}
+
 __global__ void k_hessian(DPot P, const double* __restrict__ xyz,
                           const double* __restrict__ t, int t_stride,
                           long long n, double* __restrict__ out) {
@@ -381,6 +394,15 @@ int gx_set_launch_shape(int32_t block, int32_t blocks_per_cu,
   if (block > 0) s.block = block;
   if (blocks_per_cu > 0) s.blocks_per_cu = blocks_per_cu;
   return GX_OK;
+}
+
+int gx_math_check(const double* x, int64_t n, double* out, void* stream) {
+  if (!x || !out || n < 0) return GX_EINVAL;
+  if (n == 0) return GX_OK;
+  if ((n + 255) / 256 > 0x7fffffffLL) return GX_EINVAL;
+  hipLaunchKernelGGL(k_math_check, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, x, (long long)n, out);
+  return check(hipGetLastError());
 }
 
 int gx_device_info(int32_t* num_cu, int32_t* warp_size) {

@@ -385,12 +385,7 @@ struct EvalMW {
   }
 };
 
-#ifdef GX_MW_SHARED
 using EvalMilkyWay = EvalMW;
-#else
-using EvalMilkyWay =
-    EvalFixed<GX_MIYAMOTO_NAGAI, GX_NFW, GX_HERNQUIST, GX_HERNQUIST>;
-#endif
 
 // The mock-stream field: the Milky Way model (rows 0-3) plus a Plummer
 // sphere riding a track (row 4), as MockStreamGenerator builds it with a

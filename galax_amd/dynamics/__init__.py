@@ -18,6 +18,8 @@ from ._mockstream import (
     omega,
     tidal_radius,
 )
+from . import fields
+from ._nbody import NBodyField, NBodySolution, solve_nbody
 from ._solver import (
     ConstantStepSize,
     Dopri5,
@@ -29,6 +31,10 @@ from ._solver import (
 )
 
 __all__ = [
+    "fields",
+    "NBodyField",
+    "NBodySolution",
+    "solve_nbody",
     "FardalStreamDF",
     "MockStream",
     "MockStreamArm",

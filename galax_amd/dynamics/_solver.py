@@ -115,6 +115,17 @@ class OrbitSolver:
     def with_(self, **kw) -> OrbitSolver:
         return replace(self, **kw)
 
+    def solve(self, field, qp, t0, t1, *, saveat=None, dt0=None, max_steps="default"):
+        """OrbitSolver.solve (orbit/solver.py:121-141) for an NBodyField: the
+        bodies integrate as one system; returns an NBodySolution with
+        ys = (q, p) of shape (T, N, 3)."""
+        from ._nbody import NBodyField, solve_nbody
+
+        if isinstance(field, NBodyField):
+            return solve_nbody(field, qp, t0, t1, saveat=saveat, solver=self, dt0=dt0, max_steps=max_steps)
+        raise TypeError(f"OrbitSolver.solve supports NBodyField here, not {type(field).__name__}; "
+                        "use integrate_orbits / evaluate_orbit for potentials")
+
 
 @dataclass
 class IntegrationResult:

@@ -167,7 +167,8 @@ class NullPotential(AbstractSinglePotential):
     kind = _abi.GX_NULL
 
     def __init__(self, *, units="galactic"):
-        pass
+        # Phi = 0 in any unit system; the label tells fields which G to use
+        self.units = units
 
     def _params(self):
         return ()

@@ -21,12 +21,18 @@ import numpy as np
 G = 4.498502151469553e-12
 # 1 km/s in kpc/Myr
 KMS = 1.022712165045695e-3
+# the "solarsystem" unit system (AU, yr, Msun): 1 AU in kpc (IAU 2012 AU),
+# G in AU^3 Msun^-1 yr^-2, and 1 km/s in AU/yr
+AU = 4.848136811095360e-9
+G_SOLARSYSTEM = 39.47692640889764
+KMS_SOLARSYSTEM = 0.2109495265696987
 
 _FACTORS = {
     # length -> kpc
     "kpc": 1.0,
     "pc": 1e-3,
     "Mpc": 1e3,
+    "AU": AU,
     # time -> Myr
     "Myr": 1.0,
     "Gyr": 1e3,
@@ -37,6 +43,7 @@ _FACTORS = {
     "kpc / Myr": 1.0,
     "km/s": KMS,
     "km / s": KMS,
+    "AU/yr": AU * 1e6,
     # mass -> Msun
     "Msun": 1.0,
     "solMass": 1.0,

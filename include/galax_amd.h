@@ -305,6 +305,32 @@ int gx_lockstep_stage(int32_t stage, const double* q, const double* p,
 int gx_lockstep_update(double* q, double* p, const double* G, int64_t n, double h,
                        double* t, void* stream);
 
+/* ----------------------------------------------------------------- N-body */
+/* Direct-summation N-body field with an external potential, replacing
+ * NBodyField (dynamics/_src/orbit/field_nbody.py:94-131): for bodies at
+ * xyz[n][3] with gm[n] = G m_i and softening eps,
+ *   acc_j = sum_{i != j} gm_i (x_i - x_j) / (|x_i - x_j|^2 + eps^2)^(3/2)
+ *           - grad Phi_ext(x_j, t).
+ * gx_nbody_acceleration evaluates it at one time t (any n). */
+int gx_nbody_acceleration(const gx_pot* ext, const double* xyz, const double* gm,
+                          double eps, double t, int64_t n, double* acc, void* stream);
+/* Integrates the n-body system (1 <= n <= 4096) as ONE ODE system, as
+ * OrbitSolver.solve(NBodyField, ...) does (orbit/solver.py:121-141): shared
+ * steps, the PID error norm the RMS over all 6n state components, saves at
+ * ts[T] (monotone in the direction t0 -> t1), out[n][T][6].  One status,
+ * step and accepted-step count for the system.  workspace holds
+ * gx_nbody_workspace_bytes(n) bytes. */
+int64_t gx_nbody_workspace_bytes(int64_t n);
+int gx_nbody_integrate(const gx_pot* ext, const double* gm, double eps,
+                       const gx_solver_cfg* cfg, const double* w0, double t0, double t1,
+                       const double* ts, int32_t T, int64_t n, double* out, int32_t* status,
+                       int64_t* nsteps, int64_t* naccept, void* workspace, void* stream);
+
+/* Accuracy check of the library's short FP64 primitives (no counterpart in
+ * the reference; for tests): out[4i .. 4i+3] = 1/x, 1/sqrt(x) (x > 0),
+ * log1p(x) (x > -1) and 1/x by the division routine, at x[i]. */
+int gx_math_check(const double* x, int64_t n, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,11 @@ def lib() -> ctypes.CDLL:
             _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp,
             ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp,
         ]
+        L.orc_nbody_integrate.restype = ctypes.c_int
+        L.orc_nbody_integrate.argtypes = [
+            _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_double, _vp, _vp, ctypes.c_double,
+            ctypes.c_double, _vp, ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp, _vp,
+        ]
         _lib = L
     return _lib
 
@@ -162,3 +167,25 @@ def integrate(pot, solver, w0, t0, t1, *, ts=None, T=None, dt0=None, max_steps="
     )
     assert rc == 0, rc
     return out, status, nsteps, nacc
+
+
+def nbody_integrate(ext_pot, gm, eps, solver, w0, t0, t1, *, ts=None, dt0=None, max_steps="default"):
+    """The N-body system (NBodyField) as one ODE system.  Returns
+    (out (n, T, 6), status, nsteps, naccept)."""
+    cfg = solver.cfg(dt0=dt0, max_steps=max_steps)
+    w = _c(w0).reshape(-1, 6)
+    n = w.shape[0]
+    tsa = _c(np.array([t1], dtype=np.float64) if ts is None else ts)
+    T = tsa.shape[0]
+    g = _c(gm).reshape(n)
+    out = np.empty((n, T, 6))
+    status = ctypes.c_int32()
+    nsteps = ctypes.c_int64()
+    nacc = ctypes.c_int64()
+    tab = _Table(ext_pot)
+    rc = lib().orc_nbody_integrate(
+        *tab.args(), g.ctypes.data, float(eps), ctypes.addressof(cfg), w.ctypes.data, float(t0), float(t1),
+        tsa.ctypes.data, T, n, out.ctypes.data, ctypes.byref(status), ctypes.byref(nsteps), ctypes.byref(nacc),
+    )
+    assert rc == 0, rc
+    return out, status.value, nsteps.value, nacc.value

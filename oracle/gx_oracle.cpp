@@ -557,9 +557,19 @@ const Tableau& tableau(int method) {
 // -------------------------------------------------------------- stepping
 // Vector field of Hamilton's equations in the solver's time variable
 // tau = dir * t: dy/dtau = dir * (p, -grad Phi(q, dir*tau)).
+//
+// NBodyField (dynamics/_src/orbit/field_nbody.py:94-131) couples B bodies:
+// a_j = sum_{i != j} G m_i (x_i - x_j) / (|x_i - x_j|^2 + eps^2)^(3/2)
+//       - grad Phi_ext(x_j, t).
+struct NBody {
+  std::vector<double> gm;  // G m_i
+  double eps2 = 0.0;
+};
+
 struct Field {
   const Pot* P;
   double dir;
+  const NBody* nb = nullptr;
   void operator()(double tau, const double* y, double* f) const {
     double g[3];
     gradient(*P, y, dir * tau, g);
@@ -570,6 +580,32 @@ struct Field {
     f[4] = -dir * g[1];
     f[5] = -dir * g[2];
   }
+  // the field of all B states y[6B] at once
+  void all(double tau, const double* y, int64_t B, double* f) const {
+    if (!nb) {
+      for (int64_t o = 0; o < B; ++o) (*this)(tau, y + 6 * o, f + 6 * o);
+      return;
+    }
+    for (int64_t j = 0; j < B; ++j) {
+      const double* yj = y + 6 * j;
+      double g[3];
+      gradient(*P, yj, dir * tau, g);
+      double a[3] = {-g[0], -g[1], -g[2]};
+      for (int64_t i = 0; i < B; ++i) {
+        if (i == j) continue;
+        const double* yi = y + 6 * i;
+        const double d[3] = {yi[0] - yj[0], yi[1] - yj[1], yi[2] - yj[2]};
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + nb->eps2;
+        const double w = nb->gm[i] / (r2 * std::sqrt(r2));
+        for (int m = 0; m < 3; ++m) a[m] += w * d[m];
+      }
+      double* fj = f + 6 * j;
+      for (int m = 0; m < 3; ++m) {
+        fj[m] = dir * yj[3 + m];
+        fj[3 + m] = dir * a[m];
+      }
+    }
+  }
 };
 
 // One explicit RK step of size h from (tau, y[6*B]) for B orbits sharing the
@@ -578,7 +614,7 @@ struct Field {
 void rk_step(const Tableau& T, const Field& F, double tau, const double* y,
              double h, int64_t B, double* y1, double* err) {
   const int s = T.s;
-  std::vector<double> k((size_t)s * 6 * B), Y(6 * B), fb(6);
+  std::vector<double> k((size_t)s * 6 * B), Y(6 * B), fb(6 * B);
   for (int i = 0; i < s; ++i) {
     double ti = tau + T.c[i] * h;
     for (int64_t o = 0; o < B; ++o) {
@@ -588,9 +624,10 @@ void rk_step(const Tableau& T, const Field& F, double tau, const double* y,
           if (T.a[i][j] != 0.0) acc += T.a[i][j] * k[((size_t)j * B + o) * 6 + m];
         Y[o * 6 + m] = y[o * 6 + m] + acc;
       }
-      F(ti, &Y[o * 6], fb.data());
-      for (int m = 0; m < 6; ++m) k[((size_t)i * B + o) * 6 + m] = fb[m] * h;
     }
+    F.all(ti, Y.data(), B, fb.data());
+    for (int64_t o = 0; o < B; ++o)
+      for (int m = 0; m < 6; ++m) k[((size_t)i * B + o) * 6 + m] = fb[o * 6 + m] * h;
   }
   for (int64_t o = 0; o < B; ++o)
     for (int m = 0; m < 6; ++m) {
@@ -622,7 +659,7 @@ bool all_finite(const double* x, int64_t n) {
 double initial_step(const Field& F, double tau0, const double* y0, int64_t n6,
                     double rtol, double atol, double error_order) {
   std::vector<double> f0(n6), f1(n6), y1(n6), tmp(n6), scale(n6);
-  for (int64_t i = 0; i < n6; i += 6) F(tau0, y0 + i, &f0[i]);
+  F.all(tau0, y0, n6 / 6, f0.data());
   for (int64_t i = 0; i < n6; ++i) scale[i] = atol + std::fabs(y0[i]) * rtol;
   for (int64_t i = 0; i < n6; ++i) tmp[i] = y0[i] / scale[i];
   double d0 = rms(tmp.data(), n6);
@@ -632,7 +669,7 @@ double initial_step(const Field& F, double tau0, const double* y0, int64_t n6,
   double d1_ = cond ? 1.0 : d1;
   double h0 = cond ? 1e-6 : 0.01 * (d0 / d1_);
   for (int64_t i = 0; i < n6; ++i) y1[i] = y0[i] + f0[i] * h0;
-  for (int64_t i = 0; i < n6; i += 6) F(tau0 + h0, &y1[i], &f1[i]);
+  F.all(tau0 + h0, y1.data(), n6 / 6, f1.data());
   for (int64_t i = 0; i < n6; ++i) tmp[i] = (f1[i] - f0[i]) / scale[i];
   double d2 = rms(tmp.data(), n6) / h0;
   double maxd = std::max(d1, d2);
@@ -658,7 +695,7 @@ struct SolveOut {
 // per-orbit solve).  y0[6B]; save times ts[T] in t; out[B][T][6].
 SolveOut solve(const Pot& P, const gx_solver_cfg& cfg, const double* y0,
                double t0, double t1, const double* ts, int T, int64_t B,
-               double* out) {
+               double* out, const NBody* nb = nullptr) {
   SolveOut R;
   const Tableau& Tb = tableau(cfg.method);
   const int64_t n6 = 6 * B;
@@ -673,7 +710,7 @@ SolveOut solve(const Pot& P, const gx_solver_cfg& cfg, const double* y0,
     return R;
   }
   const double dir = (t0 < t1) ? 1.0 : -1.0;
-  Field F{&P, dir};
+  Field F{&P, dir, nb};
   const double tau1 = dir * t1;
   double tau = dir * t0;
   int si = 0;
@@ -913,6 +950,28 @@ int orc_integrate(const gx_component* comps, int ncomp, const gx_track* tracks,
       if (naccept) naccept[i] = R.naccept;
     }
   }
+  return GX_OK;
+}
+
+// N-body system (NBodyField with an external potential): B bodies with
+// G m_i = gm[i] and softening eps, integrated as one system (shared steps,
+// RMS error over all 6B components) from t0 to t1 with saves at ts[T].
+int orc_nbody_integrate(const gx_component* comps, int ncomp,
+                        const gx_track* tracks, int ntracks, const double* gm,
+                        double eps, const gx_solver_cfg* cfg, const double* w0,
+                        double t0, double t1, const double* ts, int T,
+                        int64_t B, double* out, int32_t* status,
+                        int64_t* nsteps, int64_t* naccept) {
+  if (!cfg || !w0 || !gm || !ts || !out || B < 1 || T < 1) return GX_EINVAL;
+  if (cfg->controller == GX_CONSTANT && !(cfg->dt0 > 0)) return GX_EINVAL;
+  Pot P = make_pot(comps, ncomp, tracks, ntracks);
+  NBody nb;
+  nb.gm.assign(gm, gm + B);
+  nb.eps2 = eps * eps;
+  SolveOut R = solve(P, *cfg, w0, t0, t1, ts, T, B, out, &nb);
+  if (status) *status = R.status;
+  if (nsteps) *nsteps = R.nsteps;
+  if (naccept) *naccept = R.naccept;
   return GX_OK;
 }
 
