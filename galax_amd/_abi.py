@@ -15,6 +15,11 @@ GX_MIYAMOTO_NAGAI = 3
 GX_NFW = 4
 GX_PLUMMER = 5
 GX_MONARI_BAR = 6
+GX_ISOCHRONE = 7
+GX_JAFFE = 8
+GX_LOGARITHMIC = 9
+GX_LMJ09_LOG = 10
+GX_LONGMURALI_BAR = 11
 
 GX_FRAME_NONE = 0
 GX_FRAME_OFFSET = 1

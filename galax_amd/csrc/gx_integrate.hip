@@ -659,6 +659,9 @@ int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
     if (pot->sig == kSigMilkyWay)
       return pid ? launch<Dopri8, true, EvalMilkyWay>(pot, A, s)
                  : launch<Dopri8, false, EvalMilkyWay>(pot, A, s);
+    if (pot->sig == kSigMilkyWayRotating)
+      return pid ? launch<Dopri8, true, EvalMilkyWayRotating>(pot, A, s)
+                 : launch<Dopri8, false, EvalMilkyWayRotating>(pot, A, s);
     if (pot->sig == kSigMilkyWayTracked)
       return pid ? launch<Dopri8, true, EvalMilkyWayTracked>(pot, A, s)
                  : launch<Dopri8, false, EvalMilkyWayTracked>(pot, A, s);

@@ -18,10 +18,17 @@ struct gx_pot {
   int kind0 = 0;      // kind of the first component
 };
 
-// Table signatures with a compile-time evaluator: the Milky Way model, and
-// the Milky Way model plus one Plummer sphere on a track (the mock-stream
-// field with the progenitor's own potential).
-enum : int { kSigNone = 0, kSigMilkyWay = 1, kSigMilkyWayTracked = 2 };
+// Table signatures with a compile-time evaluator: the Milky Way model; the
+// Milky Way model plus one Plummer sphere on a track (the mock-stream field
+// with the progenitor's own potential); the Milky Way model plus one group
+// of perturbers rotating rigidly together (a bar and co-rotating Plummer
+// subhaloes).
+enum : int {
+  kSigNone = 0,
+  kSigMilkyWay = 1,
+  kSigMilkyWayTracked = 2,
+  kSigMilkyWayRotating = 3
+};
 
 namespace gx {
 

@@ -85,6 +85,40 @@ int derive(const gx_component& in, DComp& out) {
       out.f[5] = std::sin(2.0 * phib);
       break;
     }
+    case GX_ISOCHRONE:
+      if (!(p[1] > 0)) return GX_EINVAL;
+      out.p[0] = p[0];
+      out.p[1] = p[1];
+      out.p[2] = p[1] * p[1];
+      break;
+    case GX_JAFFE:
+      if (!(p[1] > 0)) return GX_EINVAL;
+      out.p[0] = p[0];
+      out.p[1] = p[1];
+      break;
+    case GX_LOGARITHMIC:
+      out.p[0] = p[0] * p[0];
+      out.p[1] = p[1] * p[1];
+      break;
+    case GX_LMJ09_LOG:
+      if (!(p[2] > 0 && p[3] > 0 && p[4] > 0)) return GX_EINVAL;
+      out.p[0] = p[0] * p[0];
+      out.p[1] = p[1] * p[1];
+      out.p[2] = 1.0 / (p[2] * p[2]);
+      out.p[3] = 1.0 / (p[3] * p[3]);
+      out.p[4] = 1.0 / (p[4] * p[4]);
+      out.p[5] = std::cos(p[5]);
+      out.p[6] = std::sin(p[5]);
+      break;
+    case GX_LONGMURALI_BAR:
+      if (!(p[1] > 0)) return GX_EINVAL;
+      out.p[0] = p[0] / (2.0 * p[1]);
+      out.p[1] = p[1];
+      out.p[2] = p[2];
+      out.p[3] = p[3] * p[3];
+      out.p[4] = std::cos(p[4]);
+      out.p[5] = std::sin(p[4]);
+      break;
     default:
       return GX_EINVAL;
   }
@@ -328,6 +362,17 @@ int gx_potential_create(const gx_component* comps, int32_t ncomp,
       comps[2].kind == GX_HERNQUIST && comps[3].kind == GX_HERNQUIST &&
       comps[4].kind == GX_PLUMMER && comps[4].frame == GX_FRAME_TRACK)
     h->sig = kSigMilkyWayTracked;
+  bool rot = mw_origin || ncomp > 5;
+  for (int i = 0; rot && i < 4; ++i) rot = comps[i].frame == GX_FRAME_NONE;
+  rot = rot && ncomp >= 5 && comps[0].kind == GX_MIYAMOTO_NAGAI && comps[1].kind == GX_NFW &&
+        comps[2].kind == GX_HERNQUIST && comps[3].kind == GX_HERNQUIST &&
+        (dc[4].flags & kGroupStart) && (dc[ncomp - 1].flags & kGroupEnd);
+  for (int i = 4; rot && i < ncomp; ++i) {
+    const bool bar = comps[i].kind == GX_MONARI_BAR && comps[i].frame == GX_FRAME_NONE;
+    const bool sub = comps[i].kind == GX_PLUMMER && comps[i].frame == GX_FRAME_ROTATING;
+    rot = (bar || sub) && (dc[i].flags & kInGroup) && (i == 4 || !(dc[i].flags & kGroupStart));
+  }
+  if (rot) h->sig = kSigMilkyWayRotating;
   *out = h;
   return GX_OK;
 }

@@ -125,6 +125,42 @@ __device__ inline void frame_delta(const DPot& P, const DComp& c, double t,
   }
 }
 
+// ------------------------------------------------------- dual numbers
+// Value and three partial derivatives; forward-mode differentiation of
+// gradient code (Hessians on the cold paths).
+struct Dual3 {
+  double v, d[3];
+  __device__ static Dual3 var(double x, int i) {
+    Dual3 r{x, {0.0, 0.0, 0.0}};
+    r.d[i] = 1.0;
+    return r;
+  }
+};
+__device__ inline Dual3 operator+(Dual3 a, Dual3 b) {
+  return {a.v + b.v, {a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2]}};
+}
+__device__ inline Dual3 operator-(Dual3 a, Dual3 b) {
+  return {a.v - b.v, {a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2]}};
+}
+__device__ inline Dual3 operator*(Dual3 a, Dual3 b) {
+  return {a.v * b.v,
+          {a.d[0] * b.v + a.v * b.d[0], a.d[1] * b.v + a.v * b.d[1], a.d[2] * b.v + a.v * b.d[2]}};
+}
+__device__ inline Dual3 operator*(Dual3 a, double c) { return {a.v * c, {a.d[0] * c, a.d[1] * c, a.d[2] * c}}; }
+__device__ inline Dual3 operator*(double c, Dual3 a) { return a * c; }
+__device__ inline Dual3 operator+(Dual3 a, double c) { return {a.v + c, {a.d[0], a.d[1], a.d[2]}}; }
+__device__ inline Dual3 operator-(Dual3 a, double c) { return {a.v - c, {a.d[0], a.d[1], a.d[2]}}; }
+__device__ inline Dual3 operator/(double c, Dual3 a) {
+  const double q = c / a.v, k = -q / a.v;
+  return {q, {k * a.d[0], k * a.d[1], k * a.d[2]}};
+}
+__device__ inline Dual3 operator/(Dual3 a, Dual3 b) { return a * (1.0 / b); }
+__device__ inline Dual3 sqrt(Dual3 a) {
+  const double s = ::sqrt(a.v), k = 0.5 / s;
+  return {s, {k * a.d[0], k * a.d[1], k * a.d[2]}};
+}
+using ::sqrt;  // keep the double overloads visible next to the dual one
+
 // --------------------------------------------------------- spherical g
 // For a spherical model Phi(r): a = -(Phi'(r)/r) x.  Returns Phi'(r)/r.
 __device__ __forceinline__ double sph_g(int kind, const double* p, double r2,
@@ -145,10 +181,69 @@ __device__ __forceinline__ double sph_g(int kind, const double* p, double r2,
       const double id = rsqrt_fast(r2 + p[1]);
       return p[0] * id * id * id;
     }
+    case GX_ISOCHRONE: {  // Phi'/r = G m / ((b + s)^2 s), s = sqrt(r^2 + b^2)
+      const double is = rsqrt_fast(r2 + p[2]);
+      const double u = rcp(p[1] + (r2 + p[2]) * is);
+      return p[0] * u * u * is;
+    }
+    case GX_JAFFE:  // Phi'/r = G m / (r^2 (r + r_s))
+      return p[0] * ir * ir * rcp(r + p[1]);
+    case GX_LOGARITHMIC:  // Phi'/r = v_c^2 / (r_s^2 + r^2)
+      return p[0] * rcp(p[1] + r2);
     default:
       return 0.0;
   }
 }
+
+// Triaxial logarithmic halo (LMJ09) rotated by phi about z:
+// Phi = v^2/2 log(r_s^2 + x^T M x), grad Phi = v^2 M x / D,
+// M = R^T diag(1/q1^2, 1/q2^2, 1/q3^2) R.
+__device__ __forceinline__ void lmj09_acc(const double* p, double x, double y,
+                                          double z, double& ax, double& ay,
+                                          double& az) {
+  const double c = p[5], s = p[6];
+  const double xp = x * c + y * s, yp = y * c - x * s;
+  const double mx = xp * p[2], my = yp * p[3], mz = z * p[4];  // diag(1/q^2) x'
+  const double k = p[0] * rcp(p[1] + xp * mx + yp * my + z * mz);
+  ax -= k * (mx * c - my * s);
+  ay -= k * (mx * s + my * c);
+  az -= k * mz;
+}
+
+// Gradient of the Long & Murali (1992) bar, written for any scalar type T
+// (double in the step loop; a dual number for the Hessian).  With x' along
+// the bar (angle alpha), B = b + sqrt(z^2 + c^2), T+- = sqrt((a +- x')^2 +
+// y'^2 + B^2), N = x' - a + T-, D = x' + a + T+:
+// Phi = K log(N / D), K = G m / (2a), grad Phi = K (grad N / N - grad D / D).
+template <class T>
+__device__ __forceinline__ void longmurali_grad(const double* p, T x, T y, T z,
+                                                T& gx, T& gy, T& gz) {
+  const double K = p[0], a = p[1], b = p[2], c2 = p[3], ca = p[4], sa = p[5];
+  const T xp = x * ca + y * sa, yp = y * ca - x * sa;
+  const T S = sqrt(z * z + c2);
+  const T B = S + b;
+  const T Y2 = yp * yp + B * B;
+  const T ap = xp + a, am = xp - a;
+  const T Tp = sqrt(ap * ap + Y2), Tm = sqrt(am * am + Y2);
+  const T N = am + Tm, D = ap + Tp;
+  const T iN = 1.0 / N, iD = 1.0 / D, iTp = 1.0 / Tp, iTm = 1.0 / Tm;
+  const T gxp = K * ((am * iTm + 1.0) * iN - (ap * iTp + 1.0) * iD);
+  const T gyp = K * yp * (iTm * iN - iTp * iD);
+  gz = K * (B * z / S) * (iTm * iN - iTp * iD);
+  gx = gxp * ca - gyp * sa;
+  gy = gxp * sa + gyp * ca;
+}
+
+__device__ __forceinline__ void longmurali_acc(const double* p, double x,
+                                               double y, double z, double& ax,
+                                               double& ay, double& az) {
+  double gx, gy, gz;
+  longmurali_grad<double>(p, x, y, z, gx, gy, gz);
+  ax -= gx;
+  ay -= gy;
+  az -= gz;
+}
+
 
 __device__ __forceinline__ void mn_acc(const double* p, double x, double y,
                                        double z, double& ax, double& ay,
@@ -162,6 +257,26 @@ __device__ __forceinline__ void mn_acc(const double* p, double x, double y,
   ax -= g * x;
   ay -= g * y;
   az -= g * z * A * iS;
+}
+
+// Non-spherical, non-bar models at (x, y, z) relative to their centre;
+// returns false for the spherical kinds (handled through sph_g).
+__device__ __forceinline__ bool shaped_acc(int kind, const double* p, double x,
+                                           double y, double z, double& ax,
+                                           double& ay, double& az) {
+  if (kind == GX_MIYAMOTO_NAGAI) {
+    mn_acc(p, x, y, z, ax, ay, az);
+    return true;
+  }
+  if (kind == GX_LMJ09_LOG) {
+    lmj09_acc(p, x, y, z, ax, ay, az);
+    return true;
+  }
+  if (kind == GX_LONGMURALI_BAR) {
+    longmurali_acc(p, x, y, z, ax, ay, az);
+    return true;
+  }
+  return false;
 }
 
 // Monari bar in a frame where its pattern angle is fixed: c2 = cos 2beta,
@@ -209,9 +324,7 @@ __device__ __forceinline__ void accel(const DPot& P, double x, double y,
     const DComp& c = P.comps[i];
     const int kind = c.kind;
     if (!FULL) {
-      if (kind == GX_MIYAMOTO_NAGAI) {
-        mn_acc(c.p, x, y, z, ax, ay, az);
-      } else {
+      if (!shaped_acc(kind, c.p, x, y, z, ax, ay, az)) {
         const double g = sph_g(kind, c.p, r2o, ro, iro);
         ax -= g * x;
         ay -= g * y;
@@ -232,9 +345,7 @@ __device__ __forceinline__ void accel(const DPot& P, double x, double y,
         bar_acc(c.p, c.f[4], c.f[5], xr, yr, z, gax, gay, gaz);
       } else {
         const double dx = xr - c.f[0], dy = yr - c.f[1], dz = z - c.f[2];
-        if (kind == GX_MIYAMOTO_NAGAI) {
-          mn_acc(c.p, dx, dy, dz, gax, gay, gaz);
-        } else {
+        if (!shaped_acc(kind, c.p, dx, dy, dz, gax, gay, gaz)) {
           const double r2 = dx * dx + dy * dy + dz * dz;
           const double ir = rsqrt_fast(r2 + kTiny);
           const double r = (r2 + kTiny) * ir;
@@ -252,8 +363,7 @@ __device__ __forceinline__ void accel(const DPot& P, double x, double y,
       continue;
     }
     if (c.frame == GX_FRAME_NONE) {
-      if (kind == GX_MIYAMOTO_NAGAI) {
-        mn_acc(c.p, x, y, z, ax, ay, az);
+      if (shaped_acc(kind, c.p, x, y, z, ax, ay, az)) {
       } else if (kind == GX_MONARI_BAR) {
         double s2, c2;
         sincos(c.p[5] + 2.0 * c.p[6] * t, &s2, &c2);
@@ -270,8 +380,7 @@ __device__ __forceinline__ void accel(const DPot& P, double x, double y,
       dx = x - dx;
       dy = y - dy;
       dz = z - dz;
-      if (kind == GX_MIYAMOTO_NAGAI) {
-        mn_acc(c.p, dx, dy, dz, ax, ay, az);
+      if (shaped_acc(kind, c.p, dx, dy, dz, ax, ay, az)) {
       } else if (kind == GX_MONARI_BAR) {
         double s2, c2;
         sincos(c.p[5] + 2.0 * c.p[6] * t, &s2, &c2);
@@ -386,6 +495,41 @@ struct EvalMW {
 };
 
 using EvalMilkyWay = EvalMW;
+
+// The Milky Way model (rows 0-3) plus one group of perturbers rotating
+// rigidly at a common rate w (rows 4..n-1): Monari bars at their fixed
+// co-rotating pattern angle and Plummer spheres at fixed co-rotating
+// positions -- a barred Milky Way with co-rotating subhaloes.  The point is
+// rotated once by -w t, the group summed, the sum rotated back.
+struct EvalMilkyWayRotating {
+  static constexpr bool kLarge = true;  // rolled stage loop: one copy of the group loop
+  static constexpr bool kFrames = true;
+  __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
+                                             double z, double t, double& ax,
+                                             double& ay, double& az) {
+    EvalMW::acc(P, x, y, z, t, ax, ay, az);
+    double s, c;
+    sincos(P.comps[4].f[3] * t, &s, &c);
+    const double xr = x * c + y * s, yr = y * c - x * s;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int i = 4; i < P.n; ++i) {
+      const DComp& m = P.comps[i];
+      if (m.kind == GX_MONARI_BAR) {
+        bar_acc(m.p, m.f[4], m.f[5], xr, yr, z, gx, gy, gz);
+      } else {
+        const double dx = xr - m.f[0], dy = yr - m.f[1], dz = z - m.f[2];
+        const double id = rsqrt_fast(dx * dx + dy * dy + dz * dz + m.p[1]);
+        const double g = m.p[0] * id * id * id;
+        gx -= g * dx;
+        gy -= g * dy;
+        gz -= g * dz;
+      }
+    }
+    ax += gx * c - gy * s;
+    ay += gx * s + gy * c;
+    az += gz;
+  }
+};
 
 // The mock-stream field: the Milky Way model (rows 0-3) plus a Plummer
 // sphere riding a track (row 4), as MockStreamGenerator builds it with a
@@ -505,6 +649,28 @@ __device__ inline double phi_model(const DComp& c, double x, double y,
       const double A = p[1] + sqrt(z * z + p[2]);
       return -p[0] / sqrt(x * x + y * y + A * A);
     }
+    case GX_ISOCHRONE: {
+      const double r2 = x * x + y * y + z * z + kTiny;
+      return -p[0] / (p[1] + sqrt(r2 + p[2]));
+    }
+    case GX_JAFFE: {
+      const double r = sqrt(x * x + y * y + z * z + kTiny);
+      return -(p[0] / p[1]) * log1p(p[1] / r);
+    }
+    case GX_LOGARITHMIC:
+      return 0.5 * p[0] * log(p[1] + x * x + y * y + z * z + kTiny);
+    case GX_LMJ09_LOG: {
+      const double xp = x * p[5] + y * p[6], yp = y * p[5] - x * p[6];
+      return 0.5 * p[0] * log(p[1] + xp * xp * p[2] + yp * yp * p[3] + z * z * p[4]);
+    }
+    case GX_LONGMURALI_BAR: {
+      const double a = p[1];
+      const double xp = x * p[4] + y * p[5], yp = y * p[4] - x * p[5];
+      const double B = p[2] + sqrt(z * z + p[3]);
+      const double Y2 = yp * yp + B * B;
+      const double Tp = sqrt((a + xp) * (a + xp) + Y2), Tm = sqrt((a - xp) * (a - xp) + Y2);
+      return p[0] * log((xp - a + Tm) / (xp + a + Tp));
+    }
     case GX_MONARI_BAR: {
       double s2, c2;
       sincos(p[5] + 2.0 * p[6] * t, &s2, &c2);
@@ -530,7 +696,10 @@ __device__ inline void hess_model(const DComp& c, double x, double y,
     case GX_KEPLER:
     case GX_HERNQUIST:
     case GX_NFW:
-    case GX_PLUMMER: {
+    case GX_PLUMMER:
+    case GX_ISOCHRONE:
+    case GX_JAFFE:
+    case GX_LOGARITHMIC: {
       // H = g I + (Phi'' - g) xx^T / r^2, g = Phi'/r
       const double r2 = x * x + y * y + z * z;
       const double rr = r2 + kTiny;
@@ -549,6 +718,18 @@ __device__ inline void hess_model(const DComp& c, double x, double y,
         const double u = 1.0 / (r + rs);
         d1 = GM * (L / rr - u / r);
         d2 = GM * (u / rr - 2.0 * L / (rr * r) + (2.0 * r + rs) * u * u / rr);
+      } else if (c.kind == GX_ISOCHRONE) {
+        const double sv = sqrt(rr + p[2]), u = 1.0 / (p[1] + sv);
+        d1 = GM * r * u * u / sv;
+        d2 = GM * u * u / sv * (1.0 - rr / (sv * sv) - 2.0 * rr * u / sv);
+      } else if (c.kind == GX_JAFFE) {
+        const double u = 1.0 / (r + p[1]);
+        d1 = GM * u / r;
+        d2 = -GM * (2.0 * r + p[1]) * u * u / rr;
+      } else if (c.kind == GX_LOGARITHMIC) {
+        const double D = 1.0 / (p[1] + rr);
+        d1 = GM * r * D;  // GM slot holds v_c^2
+        d2 = GM * (p[1] - rr) * D * D;
       } else {
         const double d = rr + p[1];
         const double id = 1.0 / sqrt(d);
@@ -613,6 +794,29 @@ __device__ inline void hess_model(const DComp& c, double x, double y,
           H[3 * a + b] += p[0] * (F * HW[3 * a + b] + G * X[a] * gW[b] +
                                   gW[a] * G * X[b] + W * HF);
         }
+      break;
+    }
+    case GX_LMJ09_LOG: {
+      // H = v^2 (M / D - 2 (M x)(M x)^T / D^2)
+      const double c = p[5], s = p[6], i1 = p[2], i2 = p[3], i3 = p[4];
+      const double M[9] = {c * c * i1 + s * s * i2, c * s * (i1 - i2), 0.0,
+                           c * s * (i1 - i2),       s * s * i1 + c * c * i2, 0.0,
+                           0.0,                     0.0,                     i3};
+      double Mx[3];
+      for (int a = 0; a < 3; ++a) Mx[a] = M[3 * a] * x + M[3 * a + 1] * y + M[3 * a + 2] * z;
+      const double D = 1.0 / (p[1] + x * Mx[0] + y * Mx[1] + z * Mx[2]);
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          H[3 * a + b] += p[0] * (M[3 * a + b] * D - 2.0 * Mx[a] * Mx[b] * D * D);
+      break;
+    }
+    case GX_LONGMURALI_BAR: {
+      // forward-mode derivative of the gradient
+      const Dual3 X3[3] = {Dual3::var(x, 0), Dual3::var(y, 1), Dual3::var(z, 2)};
+      Dual3 g[3];
+      longmurali_grad<Dual3>(p, X3[0], X3[1], X3[2], g[0], g[1], g[2]);
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) H[3 * a + b] += g[a].d[b];
       break;
     }
     default:

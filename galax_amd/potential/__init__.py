@@ -3,6 +3,11 @@
 from ._base import AbstractPotential, AbstractSinglePotential, Track
 from ._builtin import (
     HernquistPotential,
+    IsochronePotential,
+    JaffePotential,
+    LMJ09LogarithmicPotential,
+    LogarithmicPotential,
+    LongMuraliBarPotential,
     KeplerPotential,
     MiyamotoNagaiPotential,
     MonariEtAl2016BarPotential,
@@ -55,6 +60,11 @@ def local_circular_velocity(pot: AbstractPotential, q, t=0.0):
 
 
 __all__ = [
+    "IsochronePotential",
+    "JaffePotential",
+    "LMJ09LogarithmicPotential",
+    "LogarithmicPotential",
+    "LongMuraliBarPotential",
     "AbstractPotential",
     "AbstractSinglePotential",
     "Track",

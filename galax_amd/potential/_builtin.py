@@ -31,6 +31,11 @@ __all__ = [
     "PlummerPotential",
     "MonariEtAl2016BarPotential",
     "NullPotential",
+    "IsochronePotential",
+    "JaffePotential",
+    "LogarithmicPotential",
+    "LMJ09LogarithmicPotential",
+    "LongMuraliBarPotential",
 ]
 
 
@@ -180,3 +185,78 @@ class NullPotential(AbstractSinglePotential):
 # angle helper used by callers that build bars from degrees
 def deg(x: float) -> float:
     return math.radians(x)
+
+
+class IsochronePotential(_MassScale):
+    """Phi = -G m_tot / (r_s + sqrt(r^2 + r_s^2)) (isochrone.py:61-90)."""
+
+    kind = _abi.GX_ISOCHRONE
+
+
+class JaffePotential(_MassScale):
+    """Phi = -(G m_tot / r_s) log(1 + r_s / r) (jaffe.py:34-60)."""
+
+    kind = _abi.GX_JAFFE
+
+
+class LogarithmicPotential(AbstractSinglePotential):
+    """Phi = v_c^2 / 2 log(r_s^2 + r^2) (logarithmic.py:19-53)."""
+
+    kind = _abi.GX_LOGARITHMIC
+
+    def __init__(self, v_c, r_s, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.v_c = _p(v_c, "kpc/Myr")
+        self.r_s = _p(r_s, "kpc")
+
+    def _params(self):
+        return (self.v_c, self.r_s)
+
+    def __repr__(self):
+        return f"LogarithmicPotential(v_c={self.v_c!r}, r_s={self.r_s!r})"
+
+
+class LMJ09LogarithmicPotential(AbstractSinglePotential):
+    """Triaxial logarithmic halo of Law, Majewski & Johnston (2009):
+    Phi = v_c^2 / 2 log(r_s^2 + (x'/q1)^2 + (y'/q2)^2 + (z/q3)^2), with
+    (x', y') the position rotated by -phi about z (logarithmic.py:56-108)."""
+
+    kind = _abi.GX_LMJ09_LOG
+
+    def __init__(self, v_c, r_s, q1, q2, q3, phi, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.v_c = _p(v_c, "kpc/Myr")
+        self.r_s = _p(r_s, "kpc")
+        self.q1, self.q2, self.q3 = _p(q1), _p(q2), _p(q3)
+        self.phi = _p(phi, "rad")
+
+    def _params(self):
+        return (self.v_c, self.r_s, self.q1, self.q2, self.q3, self.phi)
+
+    def __repr__(self):
+        return (f"LMJ09LogarithmicPotential(v_c={self.v_c!r}, r_s={self.r_s!r}, q1={self.q1!r}, "
+                f"q2={self.q2!r}, q3={self.q3!r}, phi={self.phi!r})")
+
+
+class LongMuraliBarPotential(AbstractSinglePotential):
+    """Bar of Long & Murali (1992): Phi = G m / (2a) log((x' - a + T-) /
+    (x' + a + T+)), T+- = sqrt((a +- x')^2 + y'^2 + (b + sqrt(c^2 + z^2))^2),
+    (x', y') rotated by -alpha about z (longmurali.py:19-91)."""
+
+    kind = _abi.GX_LONGMURALI_BAR
+
+    def __init__(self, m_tot, a, b, c, alpha, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m_tot = _p(m_tot, "Msun")
+        self.a, self.b, self.c = _p(a, "kpc"), _p(b, "kpc"), _p(c, "kpc")
+        self.alpha = _p(alpha, "rad")
+
+    def _params(self):
+        return (G * self.m_tot, self.a, self.b, self.c, self.alpha)
+
+    def __repr__(self):
+        return (f"LongMuraliBarPotential(m_tot={self.m_tot!r}, a={self.a!r}, b={self.b!r}, c={self.c!r}, "
+                f"alpha={self.alpha!r})")

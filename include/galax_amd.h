@@ -44,6 +44,13 @@ extern "C" {
  *   GX_MONARI_BAR      p0 = alpha, p1 = R0, p2 = v0 [kpc/Myr], p3 = Rb,
  *                      p4 = phi_b [rad], p5 = Omega [rad/Myr] monari2016.py:29-138
  *   GX_NULL            no parameters (Phi = 0)              null.py
+ *   GX_ISOCHRONE       p0 = G*m_tot, p1 = r_s               isochrone.py:61-90
+ *   GX_JAFFE           p0 = G*m_tot, p1 = r_s               jaffe.py:34-60
+ *   GX_LOGARITHMIC     p0 = v_c [kpc/Myr], p1 = r_s         logarithmic.py:19-53
+ *   GX_LMJ09_LOG       p0 = v_c, p1 = r_s, p2..p4 = q1, q2, q3 (axis ratios),
+ *                      p5 = phi [rad] (rotation about z)     logarithmic.py:56-108
+ *   GX_LONGMURALI_BAR  p0 = G*m_tot, p1 = a, p2 = b, p3 = c,
+ *                      p4 = alpha [rad] (bar angle)          longmurali.py:19-91
  */
 enum gx_kind {
   GX_NULL = 0,
@@ -52,7 +59,12 @@ enum gx_kind {
   GX_MIYAMOTO_NAGAI = 3,
   GX_NFW = 4,
   GX_PLUMMER = 5,
-  GX_MONARI_BAR = 6
+  GX_MONARI_BAR = 6,
+  GX_ISOCHRONE = 7,
+  GX_JAFFE = 8,
+  GX_LOGARITHMIC = 9,
+  GX_LMJ09_LOG = 10,
+  GX_LONGMURALI_BAR = 11
 };
 
 /* Where the component sits (potential/_src/xfm/translate.py:29-199 —

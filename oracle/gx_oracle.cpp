@@ -192,6 +192,11 @@ J<O> jlog1p(const J<O>& a) {
   return chain(a, std::log1p(a.v), d, -d * d);
 }
 template <int O>
+J<O> jlog(const J<O>& a) {
+  return chain(a, std::log(a.v), 1.0 / a.v, -1.0 / (a.v * a.v));
+}
+
+template <int O>
 J<O> jcos(const J<O>& a) {
   double c = std::cos(a.v), s = std::sin(a.v);
   return chain(a, c, -s, -c);
@@ -271,6 +276,35 @@ J<O> phi_model(const gx_component& c, const J<O>& x, const J<O>& y,
       J<O> ph = jatan2(y, x);
       J<O> gam = 2.0 * (ph - phib - Om * t);
       return pref * U * (R2 / r2) * jcos(gam);
+    }
+    case GX_ISOCHRONE: {  // -G m / (r_s + sqrt(r^2 + r_s^2))
+      J<O> r = r_spherical(x, y, z);
+      return (-p[0]) / (p[1] + jsqrt(sq(r) + p[1] * p[1]));
+    }
+    case GX_JAFFE: {  // -(G m / r_s) log(1 + r_s / r)
+      J<O> r = r_spherical(x, y, z);
+      return (-p[0] / p[1]) * jlog1p(p[1] / r);
+    }
+    case GX_LOGARITHMIC: {  // v_c^2 / 2 log(r_s^2 + r^2)
+      J<O> r = r_spherical(x, y, z);
+      return (0.5 * p[0] * p[0]) * jlog(p[1] * p[1] + sq(r));
+    }
+    case GX_LMJ09_LOG: {  // v_c^2 / 2 log(r_s^2 + (x'/q1)^2 + (y'/q2)^2 + (z/q3)^2)
+      const double c = std::cos(p[5]), s = std::sin(p[5]);
+      J<O> xp = x * c + y * s;
+      J<O> yp = -(x * s) + y * c;
+      J<O> r2 = sq(xp / p[2]) + sq(yp / p[3]) + sq(z / p[4]);
+      return (0.5 * p[0] * p[0]) * jlog(p[1] * p[1] + r2);
+    }
+    case GX_LONGMURALI_BAR: {  // G m / (2a) log((x' - a + T-) / (x' + a + T+))
+      const double a = p[1], b = p[2], cc = p[3];
+      const double ca = std::cos(p[4]), sa = std::sin(p[4]);
+      J<O> xp = x * ca + y * sa;
+      J<O> yp = -(x * sa) + y * ca;
+      J<O> yz2 = sq(yp) + sq(b + jsqrt(sq(z) + cc * cc));
+      J<O> Tp = jsqrt(sq(a + xp) + yz2);
+      J<O> Tm = jsqrt(sq(a - xp) + yz2);
+      return (p[0] / (2.0 * a)) * jlog((xp - a + Tm) / (xp + a + Tp));
     }
     default:
       return jconst<O>(0.0);

@@ -17,7 +17,15 @@ def from_spec(cls_name: str, params: dict):
             phi_b=Q(params["phi_b_deg"], "deg"),
             Omega=Q(params["Omega_kms_kpc"], "km/(s kpc)"),
         )
-    return cls(**params)
+    kw = {}
+    for k, v in params.items():
+        if k.endswith("_kms"):
+            kw[k[:-4]] = Q(v, "km/s")
+        elif k.endswith("_deg"):
+            kw[k[:-4]] = Q(v, "deg")
+        else:
+            kw[k] = v
+    return cls(**kw)
 
 
 def ic(entry):
@@ -71,6 +79,12 @@ def zoo():
         "bar": bar,
         "mw": mw,
         "mw_bar_subs": gp.CompositePotential(mw=mw, bar=bar, **subs),
+        "isochrone": gp.IsochronePotential(m_tot=5e11, r_s=3.0),
+        "jaffe": gp.JaffePotential(m_tot=2e11, r_s=2.0),
+        "logarithmic": gp.LogarithmicPotential(v_c=Q(200.0, "km/s"), r_s=0.5),
+        "lmj09": gp.LMJ09LogarithmicPotential(v_c=Q(180.0, "km/s"), r_s=12.0, q1=1.38, q2=1.0, q3=1.36,
+                                             phi=Q(97.0, "deg")),
+        "longmurali": gp.LongMuraliBarPotential(m_tot=1e10, a=4.0, b=0.8, c=0.25, alpha=Q(25.0, "deg")),
         "offset_nfw": gp.TranslatedPotential(gp.NFWPotential(m=1e11, r_s=3.0), [1.0, -2.0, 0.5]),
         "offset_bar": gp.TranslatedPotential(bar, [0.5, 0.0, 0.0]),
         "track_plummer": gp.TranslatedPotential(gp.PlummerPotential(m_tot=1e9, r_s=0.3), track),

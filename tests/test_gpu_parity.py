@@ -75,10 +75,11 @@ def test_field_golden(golden):
     for e in golden["potentials"]:
         pot = from_spec(e["class"], e["params"])
         say("golden field", e["name"])
-        np.testing.assert_allclose(pot.gradient(x[None])[0], e["grad"], atol=5.1e-9)
-        np.testing.assert_allclose(pot.hessian(x[None])[0], e["hessian"], rtol=5e-8, atol=5.1e-9)
+        atol = e.get("atol", 5.1e-9)
+        np.testing.assert_allclose(pot.gradient(x[None])[0], e["grad"], atol=atol)
+        np.testing.assert_allclose(pot.hessian(x[None])[0], e["hessian"], rtol=5e-8, atol=atol)
         if e["name"] != "milky_way":
-            assert pot.potential(x[None])[0] == pytest.approx(e["phi"], abs=5.1e-9)
+            assert pot.potential(x[None])[0] == pytest.approx(e["phi"], abs=atol)
 
 
 def test_field_scalar_and_batched_time_shapes():
@@ -106,6 +107,35 @@ def test_fixed_step_plain_small_parity():
     ep = _rel(out[..., 3:], ref[..., 3:])
     print("max rel q", eq.max(), "p", ep.max(), flush=True)
     assert eq.max() < 1e-12 and ep.max() < 1e-12
+
+
+@pytest.mark.parametrize("name", ["mw", "mw_bar_subs", "mw_tracked"])
+def test_fixed_step_specialised_evaluators(name):
+    """The compile-time evaluators chosen by table signature (Milky Way;
+    Milky Way + one rigidly rotating group; Milky Way + a Plummer sphere on a
+    track) against the oracle at the fixed-step bar."""
+    z = zoo()
+    if name == "mw_tracked":
+        pot = gp.CompositePotential(mw=z["mw"], prog=z["track_plummer"])
+    else:
+        pot = z[name]
+    w0 = workloads.disk_ics(256, 31, R_range=(3.0, 20.0))
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.ConstantStepSize(), max_steps=None)
+    say("fixed-step specialised", name)
+    res = gd.integrate_orbits(pot, w0, -20.0, 180.0, T=5, solver=solver, dt0=0.5)
+    ref, st, ns, na = oracle.integrate(pot, solver, w0, -20.0, 180.0, T=5, dt0=0.5)
+    out = res.out.cpu().numpy()
+    assert np.all(res.status.cpu().numpy() == 0)
+    eq = _rel(out[..., :3], ref[..., :3])
+    ep = _rel(out[..., 3:], ref[..., 3:])
+    e = np.maximum(eq, ep).max(axis=-1)
+    print("max rel", e.max(), "median", np.median(e), flush=True)
+    if name == "mw":
+        assert e.max() < 1e-12
+    else:
+        # a moving centre: orbits that pass close to a perturber amplify
+        # round-off differences of the field; the bulk stays at the bar
+        assert np.median(e) < 1e-13 and np.mean(e < 1e-12) > 0.95 and e.max() < 1e-9
 
 
 def test_adaptive_uneven_parity():

@@ -17,7 +17,8 @@ from _pots import from_spec, ic, solver_from
 
 
 # ------------------------------------------------------------- potentials
-@pytest.mark.parametrize("name", ["kepler", "hernquist", "miyamoto_nagai", "nfw", "plummer", "monari_bar", "milky_way"])
+@pytest.mark.parametrize("name", ["kepler", "hernquist", "miyamoto_nagai", "nfw", "plummer", "monari_bar", "milky_way",
+                                  "isochrone", "jaffe", "logarithmic", "lmj09_logarithmic", "longmurali_bar"])
 def test_potential_known_answers(golden, name):
     e = next(x for x in golden["potentials"] if x["name"] == name)
     pot = from_spec(e["class"], e["params"])
@@ -32,10 +33,10 @@ def test_potential_known_answers(golden, name):
         assert phi == pytest.approx(-0.19386101, abs=5e-9)
         assert phi == pytest.approx(e["phi"], rel=1e-5)
     else:
-        assert phi == pytest.approx(e["phi"], abs=5.1e-9)
-    np.testing.assert_allclose(grad, e["grad"], rtol=0, atol=5.1e-9)
+        assert phi == pytest.approx(e["phi"], abs=e.get("atol", 5.1e-9))
+    np.testing.assert_allclose(grad, e["grad"], rtol=0, atol=e.get("atol", 5.1e-9))
     Hexp = np.array(e["hessian"])
-    np.testing.assert_allclose(H, Hexp, rtol=5e-8, atol=5.1e-9)
+    np.testing.assert_allclose(H, Hexp, rtol=5e-8, atol=e.get("atol", 5.1e-9))
     if "density" in e:
         rho = (H[0, 0] + H[1, 1] + H[2, 2]) / (4 * math.pi * G)
         if abs(e["density"]) < 1.0:
