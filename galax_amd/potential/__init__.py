@@ -8,6 +8,8 @@ from ._builtin import (
     LMJ09LogarithmicPotential,
     LogarithmicPotential,
     LongMuraliBarPotential,
+    MN3ExponentialPotential,
+    MN3Sech2Potential,
     KeplerPotential,
     MiyamotoNagaiPotential,
     MonariEtAl2016BarPotential,
@@ -17,7 +19,9 @@ from ._builtin import (
 )
 from ._composite import (
     CompositePotential,
+    LM10Potential,
     MilkyWayPotential,
+    MilkyWayPotential2022,
     RotatingTranslation,
     TimeDependentTranslationParameter,
     TranslatedPotential,
@@ -60,6 +64,10 @@ def local_circular_velocity(pot: AbstractPotential, q, t=0.0):
 
 
 __all__ = [
+    "MN3ExponentialPotential",
+    "MN3Sech2Potential",
+    "MilkyWayPotential2022",
+    "LM10Potential",
     "IsochronePotential",
     "JaffePotential",
     "LMJ09LogarithmicPotential",

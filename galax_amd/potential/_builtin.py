@@ -36,6 +36,8 @@ __all__ = [
     "LogarithmicPotential",
     "LMJ09LogarithmicPotential",
     "LongMuraliBarPotential",
+    "MN3ExponentialPotential",
+    "MN3Sech2Potential",
 ]
 
 
@@ -260,3 +262,75 @@ class LongMuraliBarPotential(AbstractSinglePotential):
     def __repr__(self):
         return (f"LongMuraliBarPotential(m_tot={self.m_tot!r}, a={self.a!r}, b={self.b!r}, c={self.c!r}, "
                 f"alpha={self.alpha!r})")
+
+
+# Smith et al. (2015, MNRAS 448, 2934) fits for three Miyamoto-Nagai disks
+# approximating an exponential disk (mn3.py:31-52): rows give the three mass
+# fractions and three a / h_R as quartics in b / h_R, for the density kept
+# positive everywhere or allowed negative far out (the better fit to 7-8
+# scale lengths); b / h_R is a cubic in h_z / h_R.
+_MN3_K_POS = (
+    (0.0036, -0.0330, 0.1117, -0.1335, 0.1749),
+    (-0.0131, 0.1090, -0.3035, 0.2921, -5.7976),
+    (-0.0048, 0.0454, -0.1425, 0.1012, 6.7120),
+    (-0.0158, 0.0993, -0.2070, -0.7089, 0.6445),
+    (-0.0319, 0.1514, -0.1279, -0.9325, 2.6836),
+    (-0.0326, 0.1816, -0.2943, -0.6329, 2.3193),
+)
+_MN3_K_NEG = (
+    (-0.0090, 0.0640, -0.1653, 0.1164, 1.9487),
+    (0.0173, -0.0903, 0.0877, 0.2029, -1.3077),
+    (-0.0051, 0.0287, -0.0361, -0.0544, 0.2242),
+    (-0.0358, 0.2610, -0.6987, -0.1193, 2.0074),
+    (-0.0830, 0.4992, -0.7967, -1.2966, 4.4441),
+    (-0.0247, 0.1718, -0.4124, -0.5944, 0.7333),
+)
+_MN3_B_EXP = (-0.269, 1.08, 1.092)
+_MN3_B_SECH2 = (-0.033, 0.262, 0.659)
+
+
+class _MN3(AbstractSinglePotential):
+    """Sum of three Miyamoto-Nagai disks approximating an exponential disk of
+    mass m_tot, radial scale h_R and vertical scale h_z; flattened into three
+    Miyamoto-Nagai rows of the component table (mn3.py:90-120)."""
+
+    _b_coeffs = _MN3_B_EXP
+
+    def __init__(self, m_tot, h_R, h_z, positive_density=False, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m_tot = _p(m_tot, "Msun")
+        self.h_R = _p(h_R, "kpc")
+        self.h_z = _p(h_z, "kpc")
+        self.positive_density = bool(positive_density)
+
+    def components(self):
+        """(masses, a's, b) of the three Miyamoto-Nagai disks."""
+        q = self.h_z / self.h_R
+        c3, c2, c1 = self._b_coeffs
+        b_hR = c3 * q**3 + c2 * q**2 + c1 * q
+        powers = (b_hR**4, b_hR**3, b_hR**2, b_hR, 1.0)
+        K = _MN3_K_POS if self.positive_density else _MN3_K_NEG
+        v = [sum(k * x for k, x in zip(row, powers)) for row in K]
+        return [f * self.m_tot for f in v[:3]], [f * self.h_R for f in v[3:]], b_hR * self.h_R
+
+    def _rows(self):
+        ms, As, b = self.components()
+        return [(_abi.component(_abi.GX_MIYAMOTO_NAGAI, (G * m, a, b)), None) for m, a in zip(ms, As)]
+
+    def __repr__(self):
+        return (f"{type(self).__name__}(m_tot={self.m_tot!r}, h_R={self.h_R!r}, h_z={self.h_z!r}, "
+                f"positive_density={self.positive_density!r})")
+
+
+class MN3ExponentialPotential(_MN3):
+    """Three Miyamoto-Nagai disks for a double-exponential disk."""
+
+    _b_coeffs = _MN3_B_EXP
+
+
+class MN3Sech2Potential(_MN3):
+    """Three Miyamoto-Nagai disks for an exponential disk with a sech^2
+    vertical profile."""
+
+    _b_coeffs = _MN3_B_SECH2

@@ -17,11 +17,19 @@ import numpy as np
 from .. import _abi
 from ..units import Q, ustrip
 from ._base import AbstractPotential, Track
-from ._builtin import HernquistPotential, MiyamotoNagaiPotential, NFWPotential
+from ._builtin import (
+    HernquistPotential,
+    LMJ09LogarithmicPotential,
+    MiyamotoNagaiPotential,
+    MN3Sech2Potential,
+    NFWPotential,
+)
 
 __all__ = [
     "CompositePotential",
     "MilkyWayPotential",
+    "MilkyWayPotential2022",
+    "LM10Potential",
     "TranslatedPotential",
     "TimeDependentTranslationParameter",
     "RotatingTranslation",
@@ -87,6 +95,44 @@ class MilkyWayPotential(CompositePotential):
             halo=_as(NFWPotential, halo, self.HALO),
             bulge=_as(HernquistPotential, bulge, self.BULGE),
             nucleus=_as(HernquistPotential, nucleus, self.NUCLEUS),
+        )
+
+
+class MilkyWayPotential2022(CompositePotential):
+    """galax's 2022 Milky Way model (milkyway.py:240-313): a three-disk
+    Miyamoto-Nagai sum for an exponential sech^2 disk, NFW halo, Hernquist
+    bulge and nucleus."""
+
+    _preset = True
+    DISK = {"m_tot": 4.7717e10, "h_R": 2.6, "h_z": 0.3, "positive_density": True}
+    HALO = {"m": 5.5427e11, "r_s": 15.626}
+    BULGE = {"m_tot": 5e9, "r_s": 1.0}
+    NUCLEUS = {"m_tot": 1.8142e9, "r_s": 0.0688867}
+
+    def __init__(self, *, disk=None, halo=None, bulge=None, nucleus=None, units="galactic"):
+        super().__init__(
+            disk=_as(MN3Sech2Potential, disk, self.DISK),
+            halo=_as(NFWPotential, halo, self.HALO),
+            bulge=_as(HernquistPotential, bulge, self.BULGE),
+            nucleus=_as(HernquistPotential, nucleus, self.NUCLEUS),
+        )
+
+
+class LM10Potential(CompositePotential):
+    """Law & Majewski (2010) Milky Way (milkyway.py:101-170): Miyamoto-Nagai
+    disk, Hernquist bulge, triaxial LMJ09 logarithmic halo."""
+
+    _preset = True
+    DISK = {"m_tot": 1e11, "a": 6.5, "b": 0.26}
+    BULGE = {"m_tot": 3.4e10, "r_s": 0.7}
+    HALO = {"v_c": Q(2.0**0.5 * 121.858, "km/s"), "r_s": 12.0, "q1": 1.38, "q2": 1.0, "q3": 1.36,
+            "phi": Q(97.0, "deg")}
+
+    def __init__(self, *, disk=None, bulge=None, halo=None, units="galactic"):
+        super().__init__(
+            disk=_as(MiyamotoNagaiPotential, disk, self.DISK),
+            bulge=_as(HernquistPotential, bulge, self.BULGE),
+            halo=_as(LMJ09LogarithmicPotential, halo, self.HALO),
         )
 
 

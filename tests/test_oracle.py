@@ -18,7 +18,8 @@ from _pots import from_spec, ic, solver_from
 
 # ------------------------------------------------------------- potentials
 @pytest.mark.parametrize("name", ["kepler", "hernquist", "miyamoto_nagai", "nfw", "plummer", "monari_bar", "milky_way",
-                                  "isochrone", "jaffe", "logarithmic", "lmj09_logarithmic", "longmurali_bar"])
+                                  "isochrone", "jaffe", "logarithmic", "lmj09_logarithmic", "longmurali_bar",
+                                  "mn3_exponential", "mn3_sech2", "milky_way_2022", "lm10"])
 def test_potential_known_answers(golden, name):
     e = next(x for x in golden["potentials"] if x["name"] == name)
     pot = from_spec(e["class"], e["params"])
