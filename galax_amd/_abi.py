@@ -20,6 +20,7 @@ GX_JAFFE = 8
 GX_LOGARITHMIC = 9
 GX_LMJ09_LOG = 10
 GX_LONGMURALI_BAR = 11
+GX_POWERLAW_CUTOFF = 12
 
 GX_FRAME_NONE = 0
 GX_FRAME_OFFSET = 1

@@ -119,6 +119,20 @@ int derive(const gx_component& in, DComp& out) {
       out.p[4] = std::cos(p[4]);
       out.p[5] = std::sin(p[4]);
       break;
+    case GX_POWERLAW_CUTOFF: {
+      const double alpha = p[1], rc = p[2];
+      if (!(alpha >= 0.0 && alpha < 2.0 && rc > 0)) return GX_EINVAL;
+      const double a1 = 1.5 - 0.5 * alpha, a0 = 1.0 - 0.5 * alpha;
+      out.p[0] = p[0];
+      out.p[1] = a1;
+      out.p[2] = 1.0 / (rc * rc);
+      out.p[3] = std::lgamma(a1);
+      out.p[4] = a0;
+      out.p[5] = std::exp(std::lgamma(a0) - std::lgamma(a1)) / rc;
+      out.p[6] = std::lgamma(a0);
+      out.p[7] = rc;
+      break;
+    }
     default:
       return GX_EINVAL;
   }

@@ -161,6 +161,39 @@ __device__ inline Dual3 sqrt(Dual3 a) {
 }
 using ::sqrt;  // keep the double overloads visible next to the dual one
 
+// Regularized lower incomplete gamma P(a, x) (a > 0, x >= 0) given
+// lga = log Gamma(a): the power series for x < a + 1, else 1 - Q from the
+// continued fraction (modified Lentz); Press et al., Numerical Recipes 6.2.
+__device__ inline double gammap(double a, double x, double lga) {
+  if (!(x > 0.0)) return 0.0;
+  const double pre = exp(a * log(x) - x - lga);
+  if (x < a + 1.0) {
+    double ap = a, del = 1.0 / a, sum = del;
+    for (int n = 0; n < 500; ++n) {
+      ap += 1.0;
+      del *= x / ap;
+      sum += del;
+      if (fabs(del) < fabs(sum) * 1e-17) break;
+    }
+    return sum * pre;
+  }
+  const double tiny = 1e-300;
+  double b = x + 1.0 - a, c = 1.0 / tiny, d = 1.0 / b, h = d;
+  for (int i = 1; i < 500; ++i) {
+    const double an = -i * (i - a);
+    b += 2.0;
+    d = an * d + b;
+    if (fabs(d) < tiny) d = tiny;
+    c = b + an / c;
+    if (fabs(c) < tiny) c = tiny;
+    d = 1.0 / d;
+    const double del = d * c;
+    h *= del;
+    if (fabs(del - 1.0) < 1e-17) break;
+  }
+  return 1.0 - pre * h;
+}
+
 // --------------------------------------------------------- spherical g
 // For a spherical model Phi(r): a = -(Phi'(r)/r) x.  Returns Phi'(r)/r.
 __device__ __forceinline__ double sph_g(int kind, const double* p, double r2,
@@ -190,6 +223,8 @@ __device__ __forceinline__ double sph_g(int kind, const double* p, double r2,
       return p[0] * ir * ir * rcp(r + p[1]);
     case GX_LOGARITHMIC:  // Phi'/r = v_c^2 / (r_s^2 + r^2)
       return p[0] * rcp(p[1] + r2);
+    case GX_POWERLAW_CUTOFF:  // Phi'/r = G M(<r) / r^3, M(<r) / M = P(a1, r^2/r_c^2)
+      return p[0] * gammap(p[1], r2 * p[2], p[3]) * ir * ir * ir;
     default:
       return 0.0;
   }
@@ -659,6 +694,10 @@ __device__ inline double phi_model(const DComp& c, double x, double y,
     }
     case GX_LOGARITHMIC:
       return 0.5 * p[0] * log(p[1] + x * x + y * y + z * z + kTiny);
+    case GX_POWERLAW_CUTOFF: {
+      const double r2 = x * x + y * y + z * z + kTiny, r = sqrt(r2), s2 = r2 * p[2];
+      return -p[0] * (gammap(p[1], s2, p[3]) / r + p[5] * (1.0 - gammap(p[4], s2, p[6])));
+    }
     case GX_LMJ09_LOG: {
       const double xp = x * p[5] + y * p[6], yp = y * p[5] - x * p[6];
       return 0.5 * p[0] * log(p[1] + xp * xp * p[2] + yp * yp * p[3] + z * z * p[4]);
@@ -699,7 +738,8 @@ __device__ inline void hess_model(const DComp& c, double x, double y,
     case GX_PLUMMER:
     case GX_ISOCHRONE:
     case GX_JAFFE:
-    case GX_LOGARITHMIC: {
+    case GX_LOGARITHMIC:
+    case GX_POWERLAW_CUTOFF: {
       // H = g I + (Phi'' - g) xx^T / r^2, g = Phi'/r
       const double r2 = x * x + y * y + z * z;
       const double rr = r2 + kTiny;
@@ -726,6 +766,11 @@ __device__ inline void hess_model(const DComp& c, double x, double y,
         const double u = 1.0 / (r + p[1]);
         d1 = GM * u / r;
         d2 = -GM * (2.0 * r + p[1]) * u * u / rr;
+      } else if (c.kind == GX_POWERLAW_CUTOFF) {
+        const double s2 = rr * p[2], P = gammap(p[1], s2, p[3]);
+        const double dP = exp((p[1] - 1.0) * log(s2) - s2 - p[3]);  // dP/d(s^2)
+        d1 = GM * P / rr;
+        d2 = GM * (dP * 2.0 * r * p[2] / rr - 2.0 * P / (rr * r));
       } else if (c.kind == GX_LOGARITHMIC) {
         const double D = 1.0 / (p[1] + rr);
         d1 = GM * r * D;  // GM slot holds v_c^2

@@ -10,6 +10,7 @@ from ._builtin import (
     LongMuraliBarPotential,
     MN3ExponentialPotential,
     MN3Sech2Potential,
+    PowerLawCutoffPotential,
     KeplerPotential,
     MiyamotoNagaiPotential,
     MonariEtAl2016BarPotential,
@@ -19,6 +20,7 @@ from ._builtin import (
 )
 from ._composite import (
     CompositePotential,
+    BovyMWPotential2014,
     LM10Potential,
     MilkyWayPotential,
     MilkyWayPotential2022,
@@ -64,6 +66,8 @@ def local_circular_velocity(pot: AbstractPotential, q, t=0.0):
 
 
 __all__ = [
+    "PowerLawCutoffPotential",
+    "BovyMWPotential2014",
     "MN3ExponentialPotential",
     "MN3Sech2Potential",
     "MilkyWayPotential2022",

@@ -38,6 +38,7 @@ __all__ = [
     "LongMuraliBarPotential",
     "MN3ExponentialPotential",
     "MN3Sech2Potential",
+    "PowerLawCutoffPotential",
 ]
 
 
@@ -334,3 +335,26 @@ class MN3Sech2Potential(_MN3):
     vertical profile."""
 
     _b_coeffs = _MN3_B_SECH2
+
+
+class PowerLawCutoffPotential(AbstractSinglePotential):
+    """Spherical power law with an exponential cut-off, rho ~ r^-alpha
+    exp(-(r/r_c)^2), normalised to m_tot (powerlawcutoff.py:31-117);
+    0 <= alpha < 2."""
+
+    kind = _abi.GX_POWERLAW_CUTOFF
+
+    def __init__(self, m_tot, alpha, r_c, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m_tot = _p(m_tot, "Msun")
+        self.alpha = _p(alpha)
+        self.r_c = _p(r_c, "kpc")
+        if not 0.0 <= self.alpha < 2.0:
+            raise ValueError("alpha must lie in [0, 2)")
+
+    def _params(self):
+        return (G * self.m_tot, self.alpha, self.r_c)
+
+    def __repr__(self):
+        return f"PowerLawCutoffPotential(m_tot={self.m_tot!r}, alpha={self.alpha!r}, r_c={self.r_c!r})"

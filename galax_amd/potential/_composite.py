@@ -23,6 +23,7 @@ from ._builtin import (
     MiyamotoNagaiPotential,
     MN3Sech2Potential,
     NFWPotential,
+    PowerLawCutoffPotential,
 )
 
 __all__ = [
@@ -30,6 +31,7 @@ __all__ = [
     "MilkyWayPotential",
     "MilkyWayPotential2022",
     "LM10Potential",
+    "BovyMWPotential2014",
     "TranslatedPotential",
     "TimeDependentTranslationParameter",
     "RotatingTranslation",
@@ -133,6 +135,23 @@ class LM10Potential(CompositePotential):
             disk=_as(MiyamotoNagaiPotential, disk, self.DISK),
             bulge=_as(HernquistPotential, bulge, self.BULGE),
             halo=_as(LMJ09LogarithmicPotential, halo, self.HALO),
+        )
+
+
+class BovyMWPotential2014(CompositePotential):
+    """Bovy (2015) MWPotential2014 (milkyway.py:34-99): Miyamoto-Nagai disk,
+    power-law bulge with an exponential cut-off, NFW halo."""
+
+    _preset = True
+    DISK = {"m_tot": 68_193_902_782.346756, "a": 3.0, "b": 0.28}
+    BULGE = {"m_tot": 4501365375.06545, "alpha": 1.8, "r_c": 1.9}
+    HALO = {"m": 4.3683325e11, "r_s": 16.0}
+
+    def __init__(self, *, disk=None, bulge=None, halo=None, units="galactic"):
+        super().__init__(
+            disk=_as(MiyamotoNagaiPotential, disk, self.DISK),
+            bulge=_as(PowerLawCutoffPotential, bulge, self.BULGE),
+            halo=_as(NFWPotential, halo, self.HALO),
         )
 
 

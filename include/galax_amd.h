@@ -51,6 +51,8 @@ extern "C" {
  *                      p5 = phi [rad] (rotation about z)     logarithmic.py:56-108
  *   GX_LONGMURALI_BAR  p0 = G*m_tot, p1 = a, p2 = b, p3 = c,
  *                      p4 = alpha [rad] (bar angle)          longmurali.py:19-91
+ *   GX_POWERLAW_CUTOFF p0 = G*m_tot, p1 = alpha (0 <= alpha < 2),
+ *                      p2 = r_c                              powerlawcutoff.py:31-117
  */
 enum gx_kind {
   GX_NULL = 0,
@@ -64,7 +66,8 @@ enum gx_kind {
   GX_JAFFE = 8,
   GX_LOGARITHMIC = 9,
   GX_LMJ09_LOG = 10,
-  GX_LONGMURALI_BAR = 11
+  GX_LONGMURALI_BAR = 11,
+  GX_POWERLAW_CUTOFF = 12
 };
 
 /* Where the component sits (potential/_src/xfm/translate.py:29-199 —

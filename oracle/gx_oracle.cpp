@@ -196,6 +196,48 @@ J<O> jlog(const J<O>& a) {
   return chain(a, std::log(a.v), 1.0 / a.v, -1.0 / (a.v * a.v));
 }
 
+// Regularized lower incomplete gamma P(a, x) for a > 0, x >= 0: the power
+// series for x < a + 1, else 1 - Q from the continued fraction (modified
+// Lentz), as in Press et al., Numerical Recipes, section 6.2.
+double gammap(double a, double x) {
+  if (x <= 0.0) return 0.0;
+  const double lg = std::lgamma(a);
+  if (x < a + 1.0) {
+    double ap = a, del = 1.0 / a, sum = del;
+    for (int n = 0; n < 1000; ++n) {
+      ap += 1.0;
+      del *= x / ap;
+      sum += del;
+      if (std::fabs(del) < std::fabs(sum) * 1e-17) break;
+    }
+    return sum * std::exp(-x + a * std::log(x) - lg);
+  }
+  const double tiny = 1e-300;
+  double b = x + 1.0 - a, c = 1.0 / tiny, d = 1.0 / b, h = d;
+  for (int i = 1; i < 1000; ++i) {
+    const double an = -i * (i - a);
+    b += 2.0;
+    d = an * d + b;
+    if (std::fabs(d) < tiny) d = tiny;
+    c = b + an / c;
+    if (std::fabs(c) < tiny) c = tiny;
+    d = 1.0 / d;
+    const double del = d * c;
+    h *= del;
+    if (std::fabs(del - 1.0) < 1e-17) break;
+  }
+  return 1.0 - std::exp(-x + a * std::log(x) - lg) * h;
+}
+
+// P(a, x) of a jet x: dP/dx = x^(a-1) e^(-x) / Gamma(a)
+template <int O>
+J<O> jgammap(double a, const J<O>& x) {
+  const double lg = std::lgamma(a);
+  const double d1 = std::exp((a - 1.0) * std::log(x.v) - x.v - lg);
+  const double d2 = d1 * ((a - 1.0) / x.v - 1.0);
+  return chain(x, gammap(a, x.v), d1, d2);
+}
+
 template <int O>
 J<O> jcos(const J<O>& a) {
   double c = std::cos(a.v), s = std::sin(a.v);
@@ -305,6 +347,17 @@ J<O> phi_model(const gx_component& c, const J<O>& x, const J<O>& y,
       J<O> Tp = jsqrt(sq(a + xp) + yz2);
       J<O> Tm = jsqrt(sq(a - xp) + yz2);
       return (p[0] / (2.0 * a)) * jlog((xp - a + Tm) / (xp + a + Tp));
+    }
+    case GX_POWERLAW_CUTOFF: {
+      // -G m [P(3/2 - alpha/2, s^2) / r + Gamma(1 - alpha/2) / Gamma(3/2 - alpha/2)
+      //        Q(1 - alpha/2, s^2) / r_c],  s = r / r_c  (the reference's form
+      // with Gamma(5/2 - alpha/2) = (3/2 - alpha/2) Gamma(3/2 - alpha/2) and
+      // Phi(inf) = 0 folded in)
+      const double a1 = 1.5 - 0.5 * p[1], a0 = 1.0 - 0.5 * p[1], rc = p[2];
+      J<O> r = r_spherical(x, y, z);
+      J<O> s2 = sq(r / rc);
+      const double k = std::exp(std::lgamma(a0) - std::lgamma(a1)) / rc;
+      return (-p[0]) * (jgammap(a1, s2) / r + k * (1.0 - jgammap(a0, s2)));
     }
     default:
       return jconst<O>(0.0);

@@ -85,6 +85,8 @@ def zoo():
         "lmj09": gp.LMJ09LogarithmicPotential(v_c=Q(180.0, "km/s"), r_s=12.0, q1=1.38, q2=1.0, q3=1.36,
                                              phi=Q(97.0, "deg")),
         "longmurali": gp.LongMuraliBarPotential(m_tot=1e10, a=4.0, b=0.8, c=0.25, alpha=Q(25.0, "deg")),
+        "powerlaw_cutoff": gp.PowerLawCutoffPotential(m_tot=4.5e9, alpha=1.8, r_c=1.9),
+        "mw2022": gp.MilkyWayPotential2022(),
         "offset_nfw": gp.TranslatedPotential(gp.NFWPotential(m=1e11, r_s=3.0), [1.0, -2.0, 0.5]),
         "offset_bar": gp.TranslatedPotential(bar, [0.5, 0.0, 0.0]),
         "track_plummer": gp.TranslatedPotential(gp.PlummerPotential(m_tot=1e9, r_s=0.3), track),

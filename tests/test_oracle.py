@@ -19,7 +19,8 @@ from _pots import from_spec, ic, solver_from
 # ------------------------------------------------------------- potentials
 @pytest.mark.parametrize("name", ["kepler", "hernquist", "miyamoto_nagai", "nfw", "plummer", "monari_bar", "milky_way",
                                   "isochrone", "jaffe", "logarithmic", "lmj09_logarithmic", "longmurali_bar",
-                                  "mn3_exponential", "mn3_sech2", "milky_way_2022", "lm10"])
+                                  "mn3_exponential", "mn3_sech2", "milky_way_2022", "lm10", "powerlaw_cutoff",
+                                  "bovy_mw_2014"])
 def test_potential_known_answers(golden, name):
     e = next(x for x in golden["potentials"] if x["name"] == name)
     pot = from_spec(e["class"], e["params"])
@@ -319,3 +320,21 @@ def test_status_codes():
     # the proposed step falls below dtmin
     _, st2, *_ = oracle.integrate(pot, s2, y0[:1], 0.0, 1000.0, dt0=200.0)
     assert st2[0] == _abi.GX_DT_MIN_REACHED
+
+
+def test_powerlaw_cutoff_against_scipy_incomplete_gamma():
+    """The oracle's incomplete gamma (series / continued fraction) against
+    scipy.special through the closed forms of Phi and Phi'(r) = G M(<r) / r^2."""
+    from scipy.special import gamma, gammainc, gammaincc
+
+    for alpha, rc in [(1.8, 1.9), (0.9, 1.0), (0.0, 3.0), (1.5, 0.2)]:
+        pot = gp.PowerLawCutoffPotential(m_tot=1e10, alpha=alpha, r_c=rc)
+        r = np.geomspace(1e-3, 200.0, 400)
+        x = np.stack([r, np.zeros_like(r), np.zeros_like(r)], -1)
+        a1, a0 = 1.5 - alpha / 2, 1.0 - alpha / 2
+        s2 = (r / rc) ** 2
+        GM = G * 1e10
+        phi = -GM * (gammainc(a1, s2) / r + gamma(a0) / gamma(a1) * gammaincc(a0, s2) / rc)
+        dphi = GM * gammainc(a1, s2) / r**2
+        np.testing.assert_allclose(oracle.evaluate(pot, "potential", x), phi, rtol=1e-12)
+        np.testing.assert_allclose(oracle.evaluate(pot, "gradient", x)[:, 0], dphi, rtol=1e-11)
