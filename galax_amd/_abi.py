@@ -21,6 +21,10 @@ GX_LOGARITHMIC = 9
 GX_LMJ09_LOG = 10
 GX_LONGMURALI_BAR = 11
 GX_POWERLAW_CUTOFF = 12
+GX_BURKERT = 13
+GX_KUZMIN = 14
+GX_SATOH = 15
+GX_STONE = 16
 
 GX_FRAME_NONE = 0
 GX_FRAME_OFFSET = 1

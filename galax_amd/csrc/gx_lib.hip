@@ -133,6 +133,27 @@ int derive(const gx_component& in, DComp& out) {
       out.p[7] = rc;
       break;
     }
+    case GX_BURKERT:
+      if (!(p[1] > 0)) return GX_EINVAL;
+      out.p[0] = p[0] / (3.0 * std::log(2.0) - M_PI / 2.0);
+      out.p[1] = p[1];
+      out.p[2] = 1.0 / p[1];
+      break;
+    case GX_KUZMIN:
+      out.p[0] = p[0];
+      out.p[1] = p[1];
+      break;
+    case GX_SATOH:
+      out.p[0] = p[0];
+      out.p[1] = p[1];
+      out.p[2] = p[2] * p[2];
+      break;
+    case GX_STONE:
+      if (!(p[1] > 0 && p[2] > p[1])) return GX_EINVAL;
+      out.p[0] = 2.0 * p[0] / (M_PI * (p[2] - p[1]));
+      out.p[1] = p[1];
+      out.p[2] = p[2];
+      break;
     default:
       return GX_EINVAL;
   }

@@ -225,6 +225,12 @@ __device__ __forceinline__ double sph_g(int kind, const double* p, double r2,
       return p[0] * rcp(p[1] + r2);
     case GX_POWERLAW_CUTOFF:  // Phi'/r = G M(<r) / r^3, M(<r) / M = P(a1, r^2/r_c^2)
       return p[0] * gammap(p[1], r2 * p[2], p[3]) * ir * ir * ir;
+    case GX_BURKERT: {  // G M(<r) / r^3, M(<r) = m/C (2 log1p x + log1p x^2 - 2 atan x)
+      const double X = r * p[2];
+      return p[0] * (2.0 * log1p(X) + log1p(X * X) - 2.0 * atan(X)) * ir * ir * ir;
+    }
+    case GX_STONE:  // G M(<r) / r^3, M(<r) ~ r_h atan(r/r_h) - r_c atan(r/r_c)
+      return p[0] * (p[2] * atan(r / p[2]) - p[1] * atan(r / p[1])) * ir * ir * ir;
     default:
       return 0.0;
   }
@@ -309,6 +315,24 @@ __device__ __forceinline__ bool shaped_acc(int kind, const double* p, double x,
   }
   if (kind == GX_LONGMURALI_BAR) {
     longmurali_acc(p, x, y, z, ax, ay, az);
+    return true;
+  }
+  if (kind == GX_KUZMIN) {  // Phi = -G m / sqrt(R^2 + (|z| + r_s)^2)
+    const double A = fabs(z) + p[1];
+    const double iD = rsqrt_fast(x * x + y * y + A * A);
+    const double g = p[0] * iD * iD * iD;
+    ax -= g * x;
+    ay -= g * y;
+    az -= g * copysign(A, z);
+    return true;
+  }
+  if (kind == GX_SATOH) {  // Phi = -G m T^(-1/2), T = R^2 + z^2 + a (a + 2 sqrt(z^2 + b^2))
+    const double S = sqrt(z * z + p[2]);
+    const double iT = rsqrt_fast(x * x + y * y + z * z + p[1] * (p[1] + 2.0 * S));
+    const double g = p[0] * iT * iT * iT;
+    ax -= g * x;
+    ay -= g * y;
+    az -= g * z * (1.0 + p[1] / S);
     return true;
   }
   return false;
@@ -694,6 +718,24 @@ __device__ inline double phi_model(const DComp& c, double x, double y,
     }
     case GX_LOGARITHMIC:
       return 0.5 * p[0] * log(p[1] + x * x + y * y + z * z + kTiny);
+    case GX_BURKERT: {
+      const double X = sqrt(x * x + y * y + z * z + kTiny) * p[2], xi = 1.0 / X;
+      const double br = M_PI - 2.0 * (1.0 + xi) * atan(X) + 2.0 * (1.0 + xi) * log1p(X) -
+                        (1.0 - xi) * log1p(X * X);
+      return -p[0] * p[2] * br;
+    }
+    case GX_KUZMIN: {
+      const double A = fabs(z) + p[1];
+      return -p[0] / sqrt(x * x + y * y + A * A);
+    }
+    case GX_SATOH:
+      return -p[0] / sqrt(x * x + y * y + z * z + p[1] * (p[1] + 2.0 * sqrt(z * z + p[2])));
+    case GX_STONE: {
+      const double r2 = x * x + y * y + z * z + kTiny, r = sqrt(r2);
+      const double rc = p[1], rh = p[2];
+      return -p[0] * ((rh * atan(r / rh) - rc * atan(r / rc)) / r +
+                      0.5 * log((r2 + rh * rh) / (r2 + rc * rc)));
+    }
     case GX_POWERLAW_CUTOFF: {
       const double r2 = x * x + y * y + z * z + kTiny, r = sqrt(r2), s2 = r2 * p[2];
       return -p[0] * (gammap(p[1], s2, p[3]) / r + p[5] * (1.0 - gammap(p[4], s2, p[6])));
@@ -739,7 +781,9 @@ __device__ inline void hess_model(const DComp& c, double x, double y,
     case GX_ISOCHRONE:
     case GX_JAFFE:
     case GX_LOGARITHMIC:
-    case GX_POWERLAW_CUTOFF: {
+    case GX_POWERLAW_CUTOFF:
+    case GX_BURKERT:
+    case GX_STONE: {
       // H = g I + (Phi'' - g) xx^T / r^2, g = Phi'/r
       const double r2 = x * x + y * y + z * z;
       const double rr = r2 + kTiny;
@@ -766,6 +810,18 @@ __device__ inline void hess_model(const DComp& c, double x, double y,
         const double u = 1.0 / (r + p[1]);
         d1 = GM * u / r;
         d2 = -GM * (2.0 * r + p[1]) * u * u / rr;
+      } else if (c.kind == GX_BURKERT) {
+        const double X = r * p[2], rs = p[1];
+        const double M = 2.0 * log1p(X) + log1p(X * X) - 2.0 * atan(X);  // G M(<r) / p0
+        d1 = GM * M / rr;
+        // M'(r) = 4 r^2 / ((r + r_s)(r^2 + r_s^2)), so G M' / r^2 = p0 4 / (...)
+        d2 = GM * (4.0 / ((r + rs) * (rr + rs * rs)) - 2.0 * M / (rr * r));
+      } else if (c.kind == GX_STONE) {
+        const double rc = p[1], rh = p[2];
+        const double M = rh * atan(r / rh) - rc * atan(r / rc);
+        const double dM = rh * rh / (rh * rh + rr) - rc * rc / (rc * rc + rr);
+        d1 = GM * M / rr;
+        d2 = GM * (dM / rr - 2.0 * M / (rr * r));
       } else if (c.kind == GX_POWERLAW_CUTOFF) {
         const double s2 = rr * p[2], P = gammap(p[1], s2, p[3]);
         const double dP = exp((p[1] - 1.0) * log(s2) - s2 - p[3]);  // dP/d(s^2)
@@ -839,6 +895,31 @@ __device__ inline void hess_model(const DComp& c, double x, double y,
           H[3 * a + b] += p[0] * (F * HW[3 * a + b] + G * X[a] * gW[b] +
                                   gW[a] * G * X[b] + W * HF);
         }
+      break;
+    }
+    case GX_KUZMIN:
+    case GX_SATOH: {
+      // Phi = -G m T^(-1/2) with T = R^2 + w(z)^2 (Kuzmin: w = |z| + r_s)
+      // or T = R^2 + z^2 + a^2 + 2 a sqrt(z^2 + b^2) (Satoh); with T_z, T_zz
+      // the z-derivatives: H_ij = G m (T_ij T^-3/2 / 2 - 3/4 T_i T_j T^-5/2)
+      double T, Tz, Tzz;
+      if (c.kind == GX_KUZMIN) {
+        const double w = fabs(z) + p[1];
+        T = x * x + y * y + w * w;
+        Tz = 2.0 * copysign(w, z);
+        Tzz = 2.0;
+      } else {
+        const double S = sqrt(z * z + p[2]);
+        T = x * x + y * y + z * z + p[1] * (p[1] + 2.0 * S);
+        Tz = 2.0 * z * (1.0 + p[1] / S);
+        Tzz = 2.0 * (1.0 + p[1] / S) - 2.0 * p[1] * z * z / (S * S * S);
+      }
+      const double iT = 1.0 / sqrt(T), i3 = iT * iT * iT, i5 = i3 * iT * iT;
+      const double Tg[3] = {2.0 * x, 2.0 * y, Tz};
+      const double Th[3] = {2.0, 2.0, Tzz};
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          H[3 * a + b] += p[0] * ((a == b ? 0.5 * Th[a] * i3 : 0.0) - 0.75 * Tg[a] * Tg[b] * i5);
       break;
     }
     case GX_LMJ09_LOG: {

@@ -11,6 +11,10 @@ from ._builtin import (
     MN3ExponentialPotential,
     MN3Sech2Potential,
     PowerLawCutoffPotential,
+    BurkertPotential,
+    KuzminPotential,
+    SatohPotential,
+    StoneOstriker15Potential,
     KeplerPotential,
     MiyamotoNagaiPotential,
     MonariEtAl2016BarPotential,
@@ -66,6 +70,10 @@ def local_circular_velocity(pot: AbstractPotential, q, t=0.0):
 
 
 __all__ = [
+    "BurkertPotential",
+    "KuzminPotential",
+    "SatohPotential",
+    "StoneOstriker15Potential",
     "PowerLawCutoffPotential",
     "BovyMWPotential2014",
     "MN3ExponentialPotential",

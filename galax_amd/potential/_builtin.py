@@ -39,6 +39,10 @@ __all__ = [
     "MN3ExponentialPotential",
     "MN3Sech2Potential",
     "PowerLawCutoffPotential",
+    "BurkertPotential",
+    "KuzminPotential",
+    "SatohPotential",
+    "StoneOstriker15Potential",
 ]
 
 
@@ -358,3 +362,69 @@ class PowerLawCutoffPotential(AbstractSinglePotential):
 
     def __repr__(self):
         return f"PowerLawCutoffPotential(m_tot={self.m_tot!r}, alpha={self.alpha!r}, r_c={self.r_c!r})"
+
+
+class BurkertPotential(AbstractSinglePotential):
+    """Burkert (1995) cored halo, rho ~ 1 / ((r + r_s)(r^2 + r_s^2)), with
+    characteristic mass m (burkert.py:150-230)."""
+
+    kind = _abi.GX_BURKERT
+
+    def __init__(self, m, r_s, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m = _p(m, "Msun")
+        self.r_s = _p(r_s, "kpc")
+
+    def _params(self):
+        return (G * self.m, self.r_s)
+
+    def __repr__(self):
+        return f"BurkertPotential(m={self.m!r}, r_s={self.r_s!r})"
+
+
+class KuzminPotential(_MassScale):
+    """Kuzmin (1956) razor-thin disk: Phi = -G m_tot / sqrt(R^2 + (|z| +
+    r_s)^2) (kuzmin.py:60-85)."""
+
+    kind = _abi.GX_KUZMIN
+
+
+class SatohPotential(AbstractSinglePotential):
+    """Satoh (1980) disk: Phi = -G m_tot / sqrt(R^2 + z^2 + a (a + 2
+    sqrt(z^2 + b^2))) (satoh.py:50-69)."""
+
+    kind = _abi.GX_SATOH
+
+    def __init__(self, m_tot, a, b, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m_tot = _p(m_tot, "Msun")
+        self.a = _p(a, "kpc")
+        self.b = _p(b, "kpc")
+
+    def _params(self):
+        return (G * self.m_tot, self.a, self.b)
+
+    def __repr__(self):
+        return f"SatohPotential(m_tot={self.m_tot!r}, a={self.a!r}, b={self.b!r})"
+
+
+class StoneOstriker15Potential(AbstractSinglePotential):
+    """Stone & Ostriker (2015) cored halo, rho ~ 1 / ((1 + r^2/r_c^2)(1 +
+    r^2/r_h^2)), total mass m_tot (stoneostriker15.py:90-147)."""
+
+    kind = _abi.GX_STONE
+
+    def __init__(self, m_tot, r_c, r_h, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m_tot = _p(m_tot, "Msun")
+        self.r_c = _p(r_c, "kpc")
+        self.r_h = _p(r_h, "kpc")
+
+    def _params(self):
+        return (G * self.m_tot, self.r_c, self.r_h)
+
+    def __repr__(self):
+        return f"StoneOstriker15Potential(m_tot={self.m_tot!r}, r_c={self.r_c!r}, r_h={self.r_h!r})"

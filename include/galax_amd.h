@@ -53,6 +53,10 @@ extern "C" {
  *                      p4 = alpha [rad] (bar angle)          longmurali.py:19-91
  *   GX_POWERLAW_CUTOFF p0 = G*m_tot, p1 = alpha (0 <= alpha < 2),
  *                      p2 = r_c                              powerlawcutoff.py:31-117
+ *   GX_BURKERT         p0 = G*m, p1 = r_s                    burkert.py:150-230
+ *   GX_KUZMIN          p0 = G*m_tot, p1 = r_s (|z| in Phi)   kuzmin.py:60-85
+ *   GX_SATOH           p0 = G*m_tot, p1 = a, p2 = b          satoh.py:50-69
+ *   GX_STONE           p0 = G*m_tot, p1 = r_c, p2 = r_h      stoneostriker15.py:90-147
  */
 enum gx_kind {
   GX_NULL = 0,
@@ -67,7 +71,11 @@ enum gx_kind {
   GX_LOGARITHMIC = 9,
   GX_LMJ09_LOG = 10,
   GX_LONGMURALI_BAR = 11,
-  GX_POWERLAW_CUTOFF = 12
+  GX_POWERLAW_CUTOFF = 12,
+  GX_BURKERT = 13,
+  GX_KUZMIN = 14,
+  GX_SATOH = 15,
+  GX_STONE = 16
 };
 
 /* Where the component sits (potential/_src/xfm/translate.py:29-199 —

@@ -239,6 +239,18 @@ J<O> jgammap(double a, const J<O>& x) {
 }
 
 template <int O>
+J<O> jatan(const J<O>& a) {
+  const double u = 1.0 / (1.0 + a.v * a.v);
+  return chain(a, std::atan(a.v), u, -2.0 * a.v * u * u);
+}
+
+template <int O>
+J<O> jabs(const J<O>& a) {  // d|x|/dx = sign(x)
+  const double s = (a.v > 0) - (a.v < 0);
+  return chain(a, std::fabs(a.v), s, 0.0);
+}
+
+template <int O>
 J<O> jcos(const J<O>& a) {
   double c = std::cos(a.v), s = std::sin(a.v);
   return chain(a, c, -s, -c);
@@ -358,6 +370,32 @@ J<O> phi_model(const gx_component& c, const J<O>& x, const J<O>& y,
       J<O> s2 = sq(r / rc);
       const double k = std::exp(std::lgamma(a0) - std::lgamma(a1)) / rc;
       return (-p[0]) * (jgammap(a1, s2) / r + k * (1.0 - jgammap(a0, s2)));
+    }
+    case GX_BURKERT: {
+      // -G m / (r_s C) [pi - 2 (1 + 1/x) atan x + 2 (1 + 1/x) log1p x
+      //                 - (1 - 1/x) log1p x^2], x = r / r_s, C = 3 log 2 - pi/2
+      const double C = 3.0 * std::log(2.0) - M_PI / 2.0;
+      J<O> X = r_spherical(x, y, z) / p[1];
+      J<O> xi = 1.0 / X;
+      J<O> br = M_PI - 2.0 * (1.0 + xi) * jatan(X) + 2.0 * (1.0 + xi) * jlog1p(X) -
+                (1.0 - xi) * jlog1p(sq(X));
+      return (-p[0] / (p[1] * C)) * br;
+    }
+    case GX_KUZMIN: {  // -G m / sqrt(R^2 + (|z| + r_s)^2)
+      return (-p[0]) / jsqrt(sq(x) + sq(y) + sq(jabs(z) + p[1]));
+    }
+    case GX_SATOH: {  // -G m / sqrt(R^2 + z^2 + a (a + 2 sqrt(z^2 + b^2)))
+      const double a = p[1], b = p[2];
+      return (-p[0]) / jsqrt(sq(x) + sq(y) + sq(z) + a * (a + 2.0 * jsqrt(sq(z) + b * b)));
+    }
+    case GX_STONE: {
+      // -2 G m / (pi (r_h - r_c)) [(r_h atan(r/r_h) - r_c atan(r/r_c)) / r
+      //                            + log((r^2 + r_h^2) / (r^2 + r_c^2)) / 2]
+      const double rc = p[1], rh = p[2];
+      J<O> r = r_spherical(x, y, z);
+      J<O> at = (rh * jatan(r / rh) - rc * jatan(r / rc)) / r;
+      J<O> lg = 0.5 * jlog((sq(r) + rh * rh) / (sq(r) + rc * rc));
+      return (-2.0 * p[0] / (M_PI * (rh - rc))) * (at + lg);
     }
     default:
       return jconst<O>(0.0);

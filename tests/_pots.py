@@ -87,6 +87,10 @@ def zoo():
         "longmurali": gp.LongMuraliBarPotential(m_tot=1e10, a=4.0, b=0.8, c=0.25, alpha=Q(25.0, "deg")),
         "powerlaw_cutoff": gp.PowerLawCutoffPotential(m_tot=4.5e9, alpha=1.8, r_c=1.9),
         "mw2022": gp.MilkyWayPotential2022(),
+        "burkert": gp.BurkertPotential(m=3e9, r_s=4.0),
+        "kuzmin": gp.KuzminPotential(m_tot=2e10, r_s=2.5),
+        "satoh": gp.SatohPotential(m_tot=5e10, a=3.0, b=0.6),
+        "stone": gp.StoneOstriker15Potential(m_tot=5e11, r_c=2.0, r_h=25.0),
         "offset_nfw": gp.TranslatedPotential(gp.NFWPotential(m=1e11, r_s=3.0), [1.0, -2.0, 0.5]),
         "offset_bar": gp.TranslatedPotential(bar, [0.5, 0.0, 0.0]),
         "track_plummer": gp.TranslatedPotential(gp.PlummerPotential(m_tot=1e9, r_s=0.3), track),
