@@ -187,9 +187,9 @@ double group_rate(const gx_component& c) {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-__global__ void k_accel(DPot P, const double* __restrict__ xyz,
-                        const double* __restrict__ t, int t_stride,
-                        long long n, double* __restrict__ out, double sign) {
+__global__ __launch_bounds__(kFieldBlock) void k_accel(
+    DPot P, const double* __restrict__ xyz, const double* __restrict__ t,
+    int t_stride, long long n, double* __restrict__ out, double sign) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += stride) {
@@ -203,9 +203,9 @@ __global__ void k_accel(DPot P, const double* __restrict__ xyz,
   }
 }
 
-__global__ void k_potential(DPot P, const double* __restrict__ xyz,
-                            const double* __restrict__ t, int t_stride,
-                            long long n, double* __restrict__ out) {
+__global__ __launch_bounds__(kFieldBlock) void k_potential(
+    DPot P, const double* __restrict__ xyz, const double* __restrict__ t,
+    int t_stride, long long n, double* __restrict__ out) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += stride) {
@@ -227,9 +227,9 @@ __global__ void k_math_check(const double* __restrict__ x, long long n,
   out[4 * i + 3] = div(1.0, v);
 }
 
-__global__ void k_hessian(DPot P, const double* __restrict__ xyz,
-                          const double* __restrict__ t, int t_stride,
-                          long long n, double* __restrict__ out) {
+__global__ __launch_bounds__(kFieldBlock) void k_hessian(
+    DPot P, const double* __restrict__ xyz, const double* __restrict__ t,
+    int t_stride, long long n, double* __restrict__ out) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += stride) {
