@@ -25,6 +25,12 @@ GX_BURKERT = 13
 GX_KUZMIN = 14
 GX_SATOH = 15
 GX_STONE = 16
+GX_TRIAXIAL_HERNQUIST = 17
+GX_TRIAXIAL_NFW = 18
+GX_LEESUTO_NFW = 19
+GX_VOGELSBERGER_NFW = 20
+
+GX_MAX_GL_ORDER = 128
 
 GX_FRAME_NONE = 0
 GX_FRAME_OFFSET = 1

@@ -9,7 +9,7 @@
 struct gx_pot {
   int device = 0;
   gx::DPot dev{};                 // pointers into `block`
-  void* block = nullptr;          // one device allocation: comps, tracks, knots
+  void* block = nullptr;  // one device allocation: comps, tracks, knots, aux
   int ncomp = 0;
   bool has_track = false;
   bool has_bar = false;

@@ -35,8 +35,42 @@ bool finite_all(const double* v, int n) {
   return true;
 }
 
+// Gauss-Legendre nodes and weights on [0, 1] (order n): Newton's method
+// on P_n from the guesses cos(pi (k - 1/4) / (n + 1/2)), P_n and P_n' by
+// the three-term recurrence; weights 2 / ((1 - x^2) P_n'(x)^2), halved for
+// the interval change.
+void gauss_legendre01(int n, std::vector<double>& s, std::vector<double>& w) {
+  s.assign(n, 0.0);
+  w.assign(n, 0.0);
+  auto legendre = [n](double x, double& pn, double& dpn) {
+    double a = 1.0, b = x;
+    for (int j = 2; j <= n; ++j) {
+      const double c = ((2 * j - 1) * x * b - (j - 1) * a) / j;
+      a = b;
+      b = c;
+    }
+    pn = n == 0 ? 1.0 : b;
+    const double pm = n == 1 ? 1.0 : a;
+    dpn = n * (x * pn - pm) / ((x - 1.0) * (x + 1.0));
+  };
+  for (int k = 0; k < n; ++k) {
+    double x = std::cos(M_PI * (k + 0.75) / (n + 0.5)), pn, dpn;
+    for (int it = 0; it < 64; ++it) {
+      legendre(x, pn, dpn);
+      const double dx = pn / dpn;
+      x -= dx;
+      if (std::fabs(dx) <= 1e-16) break;
+    }
+    legendre(x, pn, dpn);
+    s[k] = 0.5 * (x + 1.0);
+    w[k] = 1.0 / ((1.0 - x) * (1.0 + x) * dpn * dpn);
+  }
+}
+
 // Derived device constants for one component (see DComp in gx_potential.hpp).
-int derive(const gx_component& in, DComp& out) {
+// Components with a table (the triaxial NFW's quadrature nodes) append it to
+// `aux` and keep its offset in p[].
+int derive(const gx_component& in, DComp& out, std::vector<double>& aux) {
   std::memset(&out, 0, sizeof(out));
   out.kind = in.kind;
   out.frame = in.frame;
@@ -154,6 +188,57 @@ int derive(const gx_component& in, DComp& out) {
       out.p[1] = p[1];
       out.p[2] = p[2];
       break;
+    case GX_TRIAXIAL_HERNQUIST:
+      if (!(p[2] > 0 && p[3] > 0)) return GX_EINVAL;
+      out.p[0] = p[0];
+      out.p[1] = p[1];
+      out.p[2] = 1.0 / (p[2] * p[2]);
+      out.p[3] = 1.0 / (p[3] * p[3]);
+      break;
+    case GX_TRIAXIAL_NFW: {
+      const double rs = p[1], q1 = p[2], q2 = p[3];
+      const int n = (int)p[4];
+      if (!(rs > 0 && q1 > 0 && q2 > 0) || p[4] != (double)n || n < 1 ||
+          n > GX_MAX_GL_ORDER)
+        return GX_EINVAL;
+      std::vector<double> sk, wk;
+      gauss_legendre01(n, sk, wk);
+      out.p[0] = p[0] * q1 * q2;
+      out.p[1] = rs;
+      out.p[2] = (double)aux.size();
+      out.p[3] = n;
+      for (int k = 0; k < n; ++k) {
+        const double s2 = sk[k] * sk[k];
+        const double A1 = 1.0 + (q1 * q1 - 1.0) * s2, A2 = 1.0 + (q2 * q2 - 1.0) * s2;
+        aux.push_back(s2);
+        aux.push_back(s2 / A1);
+        aux.push_back(s2 / A2);
+        aux.push_back(wk[k] / std::sqrt(A1 * A2));
+      }
+      break;
+    }
+    case GX_LEESUTO_NFW: {
+      const double rs = p[1], a1 = p[2], a2 = p[3], a3 = p[4];
+      if (!(rs > 0 && a3 > 0 && a1 >= a2 && a2 >= a3)) return GX_EINVAL;
+      const double eb2 = 1.0 - (a2 / a1) * (a2 / a1), ec2 = 1.0 - (a3 / a1) * (a3 / a1);
+      const double ln2 = std::log(2.0);
+      out.p[0] = (p[0] / rs) / (ln2 - 0.5 + (ln2 - 0.75) * (eb2 + ec2));
+      out.p[1] = 1.0 / rs;
+      out.p[2] = eb2;
+      out.p[3] = ec2;
+      out.p[4] = 0.5 * (eb2 + ec2);
+      break;
+    }
+    case GX_VOGELSBERGER_NFW: {
+      const double rs = p[1], q1sq = p[2] * p[2];
+      if (!(rs > 0 && q1sq > 0 && q1sq < 3.0)) return GX_EINVAL;
+      out.p[0] = p[0];
+      out.p[1] = rs;
+      out.p[2] = 1.0 / q1sq;
+      out.p[3] = 1.0 / (3.0 - q1sq);
+      out.p[4] = p[3] * rs;
+      break;
+    }
     default:
       return GX_EINVAL;
   }
@@ -272,10 +357,11 @@ int gx_potential_create(const gx_component* comps, int32_t ncomp,
     return GX_EINVAL;
   *out = nullptr;
   std::vector<DComp> dc(ncomp > 0 ? ncomp : 1);
+  std::vector<double> aux;
   bool has_track = false, has_bar = false;
   for (int i = 0; i < ncomp; ++i) {
     if (comps[i].reserved != 0) return GX_EINVAL;
-    if (derive(comps[i], dc[i]) != GX_OK) return GX_EINVAL;
+    if (derive(comps[i], dc[i], aux) != GX_OK) return GX_EINVAL;
     if (comps[i].frame == GX_FRAME_TRACK) {
       if (comps[i].track < 0 || comps[i].track >= ntracks) return GX_EINVAL;
       has_track = true;
@@ -316,7 +402,8 @@ int gx_potential_create(const gx_component* comps, int32_t ncomp,
   }
   const size_t comp_bytes = align256(sizeof(DComp) * dc.size());
   const size_t track_bytes = align256(sizeof(DTrack) * (ntracks > 0 ? ntracks : 1));
-  const size_t total = comp_bytes + track_bytes + knot_bytes;
+  const size_t aux_bytes = align256(sizeof(double) * (aux.empty() ? 1 : aux.size()));
+  const size_t total = comp_bytes + track_bytes + knot_bytes + aux_bytes;
 
   gx_pot* h = new gx_pot();
   if (hipGetDevice(&h->device) != hipSuccess) {
@@ -368,6 +455,8 @@ int gx_potential_create(const gx_component* comps, int32_t ncomp,
     dt[k] = d;
   }
   std::memcpy(host.data() + comp_bytes, dt.data(), sizeof(DTrack) * dt.size());
+  if (!aux.empty())
+    std::memcpy(host.data() + off, aux.data(), sizeof(double) * aux.size());
   if (hipMemcpy(h->block, host.data(), total, hipMemcpyHostToDevice) !=
       hipSuccess) {
     (void)hipFree(h->block);
@@ -376,6 +465,7 @@ int gx_potential_create(const gx_component* comps, int32_t ncomp,
   }
   h->dev.comps = reinterpret_cast<const DComp*>(base);
   h->dev.tracks = reinterpret_cast<const DTrack*>(base + comp_bytes);
+  h->dev.aux = reinterpret_cast<const double*>(base + off);
   h->dev.n = ncomp;
   h->dev.ntracks = ntracks;
   h->ncomp = ncomp;

@@ -69,6 +69,7 @@ struct DTrack {
 struct DPot {
   const DComp* comps;
   const DTrack* tracks;
+  const double* aux;  // per-component tables (quadrature nodes), by offset
   int n;
   int ntracks;
 };
@@ -159,7 +160,15 @@ __device__ inline Dual3 sqrt(Dual3 a) {
   const double s = ::sqrt(a.v), k = 0.5 / s;
   return {s, {k * a.d[0], k * a.d[1], k * a.d[2]}};
 }
-using ::sqrt;  // keep the double overloads visible next to the dual one
+__device__ inline Dual3 operator-(Dual3 a) { return {-a.v, {-a.d[0], -a.d[1], -a.d[2]}}; }
+__device__ inline Dual3 operator+(double c, Dual3 a) { return a + c; }
+__device__ inline Dual3 operator-(double c, Dual3 a) { return (-a) + c; }
+__device__ inline Dual3 log1p(Dual3 a) {
+  const double k = 1.0 / (1.0 + a.v);
+  return {::log1p(a.v), {k * a.d[0], k * a.d[1], k * a.d[2]}};
+}
+using ::log1p;
+using ::sqrt;  // keep the double overloads visible next to the dual ones
 
 // Regularized lower incomplete gamma P(a, x) (a > 0, x >= 0) given
 // lga = log Gamma(a): the power series for x < a + 1, else 1 - Q from the
@@ -300,9 +309,108 @@ __device__ __forceinline__ void mn_acc(const double* p, double x, double y,
   az -= g * z * A * iS;
 }
 
+// Triaxial Hernquist: Phi = -G m / (r' + r_s), r'^2 = x^2 + y^2/q1^2 +
+// z^2/q2^2 (+ the smallest normal double); p = {G m, r_s, 1/q1^2, 1/q2^2}.
+// Returns grad Phi (T = double, or Dual3 for the Hessian).
+template <class T>
+__device__ __forceinline__ void thernquist_grad(const double* p, T x, T y, T z,
+                                                T& gx, T& gy, T& gz) {
+  const T wy = y * p[2], wz = z * p[3];
+  const T r = sqrt(x * x + y * wy + z * wz + kTiny);
+  const T d = r + p[1];
+  const T g = p[0] / (d * d * r);
+  gx = g * x;
+  gy = g * wy;
+  gz = g * wz;
+}
+
+// Vogelsberger et al. (2008): Phi = -G m log(1 + r~/r_s) / r~ with
+// r~ = (r_a + r) r_e / (r_a + r_e), r_e^2 = x^2 + y^2/q1^2 + z^2/q2^2;
+// p = {G m, r_s, 1/q1^2, 1/q2^2, r_a}.  grad Phi = Phi'(r~) grad r~.
+template <class T>
+__device__ __forceinline__ void vogelsberger_grad(const double* p, T x, T y,
+                                                  T z, T& gx, T& gy, T& gz) {
+  const double rs = p[1], ra = p[4];
+  const T wy = y * p[2], wz = z * p[3];
+  const T re = sqrt(x * x + y * wy + z * wz + kTiny);
+  const T r = sqrt(x * x + y * y + z * z + kTiny);
+  const T ie = 1.0 / (re + ra);
+  const T rt = (r + ra) * re * ie;
+  const T irt = 1.0 / rt;
+  const T d1 = p[0] * (log1p(rt * (1.0 / rs)) * irt * irt - irt / (rt + rs));
+  const T kr = d1 * re * ie / r;                    // dPhi/dr / r
+  const T ke = d1 * (r + ra) * ra * ie * ie / re;   // dPhi/dr_e / r_e
+  gx = (kr + ke) * x;
+  gy = kr * y + ke * wy;
+  gz = kr * z + ke * wz;
+}
+
+// Lee & Suto (2003): Phi = phi0 [F1(s) + E F2(s) + B F3(s)], s = r/r_s,
+// E = (e_b^2 + e_c^2)/2, B = (e_b^2 y^2 + e_c^2 z^2) / (2 r^2);
+// p = {phi0, 1/r_s, e_b^2, e_c^2, E}.  With L = log(1 + s):
+//   F1 = -L/s, F2 = -1/(2s) + 1/s^2 + (1/s - 1/s^3) L,
+//   F3 = (s^2 - 3s - 6) / (2 s^2 (1 + s)) + 3 L / s^3,
+// and grad B = (e . x - 2 B x) / r^2 with e = (0, e_b^2, e_c^2).
+template <class T>
+__device__ __forceinline__ void leesuto_grad(const double* p, T x, T y, T z,
+                                             T& gx, T& gy, T& gz) {
+  const T r2 = x * x + y * y + z * z + kTiny;
+  const T r = sqrt(r2);
+  const T ir2 = 1.0 / r2;
+  const T s = r * p[1];
+  const T is = 1.0 / s;
+  const T iu = 1.0 / (s + 1.0);
+  const T L = log1p(s);
+  const T is2 = is * is, is3 = is2 * is, is4 = is2 * is2;
+  const T F1p = L * is2 - is * iu;
+  const T F2p = 0.5 * is2 - 2.0 * is3 + (3.0 * is4 - is2) * L + (is - is3) * iu;
+  const T F3 = (s * s - 3.0 * s - 6.0) * 0.5 * is2 * iu + 3.0 * is3 * L;
+  const T F3p = (((6.0 - s) * s + 21.0) * s + 12.0) * 0.5 * is3 * iu * iu +
+                3.0 * is3 * iu - 9.0 * is4 * L;
+  const T B = (p[2] * (y * y) + p[3] * (z * z)) * 0.5 * ir2;
+  const T Fs = F1p + p[4] * F2p + B * F3p;
+  const T kr = p[0] * (Fs * p[1] / r - 2.0 * B * F3 * ir2);
+  const T ka = p[0] * F3 * ir2;
+  gx = kr * x;
+  gy = (kr + ka * p[2]) * y;
+  gz = (kr + ka * p[3]) * z;
+}
+
+// Triaxial NFW density by Chandrasekhar's integral over ellipsoidal
+// shells, on a Gauss-Legendre rule: Phi = -P sum_k w_k / (r_s + u_k),
+// u_k^2 = a_k x^2 + b_k y^2 + c_k z^2, so grad Phi = P sum_k w_k
+// (a_k x, b_k y, c_k z) / (u_k (r_s + u_k)^2).  p = {P = G m q1 q2, r_s,
+// offset of the node table in aux, nodes}; node k holds {a_k = s_k^2,
+// b_k = s_k^2 / (1 + (q1^2 - 1) s_k^2), c_k (same with q2), w_k / sqrt((1 +
+// (q1^2 - 1) s_k^2)(1 + (q2^2 - 1) s_k^2))}.  The node loop reads one
+// uniform table (scalar loads).
+__device__ __forceinline__ void tnfw_acc(const double* p, const double* aux,
+                                         double x, double y, double z,
+                                         double& ax, double& ay, double& az) {
+  const double* tab = aux + (long long)p[2];
+  const int n = (int)p[3];
+  const double rs = p[1];
+  const double x2 = x * x, y2 = y * y, z2 = z * z;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const double a = tab[4 * k], b = tab[4 * k + 1], c = tab[4 * k + 2];
+    const double S = fma(a, x2, fma(b, y2, fma(c, z2, kTiny)));
+    const double iu = rsqrt_fast(S);
+    const double d = fma(S, iu, rs);
+    const double g = tab[4 * k + 3] * iu * rcp(d * d);
+    sx = fma(g, a, sx);
+    sy = fma(g, b, sy);
+    sz = fma(g, c, sz);
+  }
+  ax -= p[0] * sx * x;
+  ay -= p[0] * sy * y;
+  az -= p[0] * sz * z;
+}
+
 // Non-spherical, non-bar models at (x, y, z) relative to their centre;
 // returns false for the spherical kinds (handled through sph_g).
-__device__ __forceinline__ bool shaped_acc(int kind, const double* p, double x,
+__device__ __forceinline__ bool shaped_acc(int kind, const double* p,
+                                           const double* aux, double x,
                                            double y, double z, double& ax,
                                            double& ay, double& az) {
   if (kind == GX_MIYAMOTO_NAGAI) {
@@ -333,6 +441,23 @@ __device__ __forceinline__ bool shaped_acc(int kind, const double* p, double x,
     ax -= g * x;
     ay -= g * y;
     az -= g * z * (1.0 + p[1] / S);
+    return true;
+  }
+  if (kind == GX_TRIAXIAL_NFW) {
+    tnfw_acc(p, aux, x, y, z, ax, ay, az);
+    return true;
+  }
+  if (kind >= GX_TRIAXIAL_HERNQUIST && kind <= GX_VOGELSBERGER_NFW) {
+    double gx, gy, gz;
+    if (kind == GX_TRIAXIAL_HERNQUIST)
+      thernquist_grad<double>(p, x, y, z, gx, gy, gz);
+    else if (kind == GX_LEESUTO_NFW)
+      leesuto_grad<double>(p, x, y, z, gx, gy, gz);
+    else
+      vogelsberger_grad<double>(p, x, y, z, gx, gy, gz);
+    ax -= gx;
+    ay -= gy;
+    az -= gz;
     return true;
   }
   return false;
@@ -383,7 +508,7 @@ __device__ __forceinline__ void accel(const DPot& P, double x, double y,
     const DComp& c = P.comps[i];
     const int kind = c.kind;
     if (!FULL) {
-      if (!shaped_acc(kind, c.p, x, y, z, ax, ay, az)) {
+      if (!shaped_acc(kind, c.p, P.aux, x, y, z, ax, ay, az)) {
         const double g = sph_g(kind, c.p, r2o, ro, iro);
         ax -= g * x;
         ay -= g * y;
@@ -404,7 +529,7 @@ __device__ __forceinline__ void accel(const DPot& P, double x, double y,
         bar_acc(c.p, c.f[4], c.f[5], xr, yr, z, gax, gay, gaz);
       } else {
         const double dx = xr - c.f[0], dy = yr - c.f[1], dz = z - c.f[2];
-        if (!shaped_acc(kind, c.p, dx, dy, dz, gax, gay, gaz)) {
+        if (!shaped_acc(kind, c.p, P.aux, dx, dy, dz, gax, gay, gaz)) {
           const double r2 = dx * dx + dy * dy + dz * dz;
           const double ir = rsqrt_fast(r2 + kTiny);
           const double r = (r2 + kTiny) * ir;
@@ -422,7 +547,7 @@ __device__ __forceinline__ void accel(const DPot& P, double x, double y,
       continue;
     }
     if (c.frame == GX_FRAME_NONE) {
-      if (shaped_acc(kind, c.p, x, y, z, ax, ay, az)) {
+      if (shaped_acc(kind, c.p, P.aux, x, y, z, ax, ay, az)) {
       } else if (kind == GX_MONARI_BAR) {
         double s2, c2;
         sincos(c.p[5] + 2.0 * c.p[6] * t, &s2, &c2);
@@ -439,7 +564,7 @@ __device__ __forceinline__ void accel(const DPot& P, double x, double y,
       dx = x - dx;
       dy = y - dy;
       dz = z - dz;
-      if (shaped_acc(kind, c.p, dx, dy, dz, ax, ay, az)) {
+      if (shaped_acc(kind, c.p, P.aux, dx, dy, dz, ax, ay, az)) {
       } else if (kind == GX_MONARI_BAR) {
         double s2, c2;
         sincos(c.p[5] + 2.0 * c.p[6] * t, &s2, &c2);
@@ -689,10 +814,36 @@ __device__ __forceinline__ void comp_adjoint(const DComp& c, double x,
 
 // ------------------------------------------------ potential and Hessian
 // (not on the integration hot path: each component in the inertial frame)
-__device__ inline double phi_model(const DComp& c, double x, double y,
-                                   double z, double t) {
+__device__ inline double phi_model(const DComp& c, const double* aux,
+                                   double x, double y, double z, double t) {
   const double* p = c.p;
   switch (c.kind) {
+    case GX_TRIAXIAL_HERNQUIST:
+      return -p[0] / (sqrt(x * x + y * y * p[2] + z * z * p[3] + kTiny) + p[1]);
+    case GX_TRIAXIAL_NFW: {
+      const double* tab = aux + (long long)p[2];
+      double sum = 0.0;
+      for (int k = 0; k < (int)p[3]; ++k) {
+        const double S = tab[4 * k] * x * x + tab[4 * k + 1] * y * y + tab[4 * k + 2] * z * z;
+        sum += tab[4 * k + 3] / (p[1] + sqrt(S + kTiny));
+      }
+      return -p[0] * sum;
+    }
+    case GX_LEESUTO_NFW: {
+      const double r2 = x * x + y * y + z * z + kTiny, r = sqrt(r2), s = r * p[1];
+      const double L = log1p(s), is = 1.0 / s;
+      const double F1 = -L * is;
+      const double F2 = -0.5 * is + is * is + (is - is * is * is) * L;
+      const double F3 = (s * s - 3.0 * s - 6.0) / (2.0 * s * s * (1.0 + s)) + 3.0 * is * is * is * L;
+      const double B = (p[2] * y * y + p[3] * z * z) * 0.5 / r2;
+      return p[0] * (F1 + p[4] * F2 + B * F3);
+    }
+    case GX_VOGELSBERGER_NFW: {
+      const double re = sqrt(x * x + y * y * p[2] + z * z * p[3] + kTiny);
+      const double r = sqrt(x * x + y * y + z * z + kTiny);
+      const double rt = (p[4] + r) * re / (p[4] + re);
+      return -p[0] * log1p(rt / p[1]) / rt;
+    }
     case GX_KEPLER:
     case GX_HERNQUIST:
     case GX_NFW:
@@ -769,11 +920,46 @@ __device__ inline double phi_model(const DComp& c, double x, double y,
 
 // Hessian of one component at (x, y, z) (relative coordinates), added to H
 // (row-major 3x3).
-__device__ inline void hess_model(const DComp& c, double x, double y,
-                                  double z, double t, double* H) {
+__device__ inline void hess_model(const DComp& c, const double* aux,
+                                  double x, double y, double z, double t,
+                                  double* H) {
   const double* p = c.p;
   const double X[3] = {x, y, z};
   switch (c.kind) {
+    case GX_TRIAXIAL_NFW: {
+      // H_ij = P sum_k w_k [delta_ij c_i g(u) + c_i c_j x_i x_j g'(u)/u],
+      // g = 1/(u (r_s + u)^2), g'(u)/u = -(r_s + 3u) / (u^3 (r_s + u)^3)
+      const double* tab = aux + (long long)p[2];
+      const double rs = p[1];
+      for (int k = 0; k < (int)p[3]; ++k) {
+        const double C[3] = {tab[4 * k], tab[4 * k + 1], tab[4 * k + 2]};
+        const double u = sqrt(C[0] * x * x + C[1] * y * y + C[2] * z * z + kTiny);
+        const double d = rs + u;
+        const double w = p[0] * tab[4 * k + 3];
+        const double g = 1.0 / (u * d * d);
+        const double h = -(rs + 3.0 * u) / (u * u * u * d * d * d);
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b)
+            H[3 * a + b] += w * ((a == b ? C[a] * g : 0.0) + C[a] * C[b] * X[a] * X[b] * h);
+      }
+      break;
+    }
+    case GX_TRIAXIAL_HERNQUIST:
+    case GX_LEESUTO_NFW:
+    case GX_VOGELSBERGER_NFW: {
+      // forward-mode derivative of the gradient
+      const Dual3 X3[3] = {Dual3::var(x, 0), Dual3::var(y, 1), Dual3::var(z, 2)};
+      Dual3 g[3];
+      if (c.kind == GX_TRIAXIAL_HERNQUIST)
+        thernquist_grad<Dual3>(p, X3[0], X3[1], X3[2], g[0], g[1], g[2]);
+      else if (c.kind == GX_LEESUTO_NFW)
+        leesuto_grad<Dual3>(p, X3[0], X3[1], X3[2], g[0], g[1], g[2]);
+      else
+        vogelsberger_grad<Dual3>(p, X3[0], X3[1], X3[2], g[0], g[1], g[2]);
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) H[3 * a + b] += g[a].d[b];
+      break;
+    }
     case GX_KEPLER:
     case GX_HERNQUIST:
     case GX_NFW:
@@ -957,7 +1143,7 @@ __device__ inline double potential(const DPot& P, double x, double y,
     const DComp& c = P.comps[i];
     double dx, dy, dz;
     frame_delta(P, c, t, dx, dy, dz);
-    s += phi_model(c, x - dx, y - dy, z - dz, t);
+    s += phi_model(c, P.aux, x - dx, y - dy, z - dz, t);
   }
   return s;
 }
@@ -969,7 +1155,7 @@ __device__ inline void hessian(const DPot& P, double x, double y, double z,
     const DComp& c = P.comps[i];
     double dx, dy, dz;
     frame_delta(P, c, t, dx, dy, dz);
-    hess_model(c, x - dx, y - dy, z - dz, t, H);
+    hess_model(c, P.aux, x - dx, y - dy, z - dz, t, H);
   }
 }
 

@@ -15,6 +15,10 @@ from ._builtin import (
     KuzminPotential,
     SatohPotential,
     StoneOstriker15Potential,
+    TriaxialHernquistPotential,
+    TriaxialNFWPotential,
+    LeeSutoTriaxialNFWPotential,
+    Vogelsberger08TriaxialNFWPotential,
     KeplerPotential,
     MiyamotoNagaiPotential,
     MonariEtAl2016BarPotential,
@@ -74,6 +78,10 @@ __all__ = [
     "KuzminPotential",
     "SatohPotential",
     "StoneOstriker15Potential",
+    "TriaxialHernquistPotential",
+    "TriaxialNFWPotential",
+    "LeeSutoTriaxialNFWPotential",
+    "Vogelsberger08TriaxialNFWPotential",
     "PowerLawCutoffPotential",
     "BovyMWPotential2014",
     "MN3ExponentialPotential",

@@ -11,6 +11,8 @@ them to a component-table row (include/galax_amd.h):
   MonariEtAl2016BarPotential(...)     builtin/monari2016.py:29-138
   NullPotential()                     builtin/null.py
 
+and the further models below, each citing its reference file.
+
 Parameters are numbers in galactic units (kpc, Myr, Msun, rad) or `Q`
 quantities.  Mass parameters enter the table as G*m.
 """
@@ -43,6 +45,10 @@ __all__ = [
     "KuzminPotential",
     "SatohPotential",
     "StoneOstriker15Potential",
+    "TriaxialHernquistPotential",
+    "TriaxialNFWPotential",
+    "LeeSutoTriaxialNFWPotential",
+    "Vogelsberger08TriaxialNFWPotential",
 ]
 
 
@@ -428,3 +434,107 @@ class StoneOstriker15Potential(AbstractSinglePotential):
 
     def __repr__(self):
         return f"StoneOstriker15Potential(m_tot={self.m_tot!r}, r_c={self.r_c!r}, r_h={self.r_h!r})"
+
+
+class TriaxialHernquistPotential(AbstractSinglePotential):
+    """Hernquist profile of the ellipsoidal radius r' = sqrt(x^2 + (y/q1)^2 +
+    (z/q2)^2): Phi = -G m_tot / (r' + r_s) (hernquist.py:89-182)."""
+
+    kind = _abi.GX_TRIAXIAL_HERNQUIST
+
+    def __init__(self, m_tot, r_s, q1=1.0, q2=1.0, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m_tot = _p(m_tot, "Msun")
+        self.r_s = _p(r_s, "kpc")
+        self.q1, self.q2 = _p(q1), _p(q2)
+        if not (self.q1 > 0 and self.q2 > 0):
+            raise ValueError("q1 and q2 must be positive")
+
+    def _params(self):
+        return (G * self.m_tot, self.r_s, self.q1, self.q2)
+
+    def __repr__(self):
+        return (f"TriaxialHernquistPotential(m_tot={self.m_tot!r}, r_s={self.r_s!r}, "
+                f"q1={self.q1!r}, q2={self.q2!r})")
+
+
+class TriaxialNFWPotential(AbstractSinglePotential):
+    """NFW density of the ellipsoidal radius (x^2 + y^2/q1^2 + z^2/q2^2)^(1/2):
+    its potential by Chandrasekhar's integral over ellipsoidal shells, taken
+    with an `integration_order`-point Gauss-Legendre rule on [0, 1]
+    (nfw/triaxial.py:29-230; the rule is the reference's
+    GaussLegendreIntegrator, potential/_src/utils.py:131-162)."""
+
+    kind = _abi.GX_TRIAXIAL_NFW
+
+    def __init__(self, m, r_s, q1=1.0, q2=1.0, *, integration_order=50, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m = _p(m, "Msun")
+        self.r_s = _p(r_s, "kpc")
+        self.q1, self.q2 = _p(q1), _p(q2)
+        self.integration_order = int(integration_order)
+        if not (self.q1 > 0 and self.q2 > 0 and self.r_s > 0):
+            raise ValueError("q1, q2 and r_s must be positive")
+        if not 1 <= self.integration_order <= _abi.GX_MAX_GL_ORDER:
+            raise ValueError(f"integration_order must be in [1, {_abi.GX_MAX_GL_ORDER}]")
+
+    def _params(self):
+        return (G * self.m, self.r_s, self.q1, self.q2, float(self.integration_order))
+
+    def __repr__(self):
+        return (f"TriaxialNFWPotential(m={self.m!r}, r_s={self.r_s!r}, q1={self.q1!r}, "
+                f"q2={self.q2!r}, integration_order={self.integration_order})")
+
+
+class LeeSutoTriaxialNFWPotential(AbstractSinglePotential):
+    """Lee & Suto (2003) approximation to the potential of a triaxial NFW
+    density, first order in the eccentricities e_b^2 = 1 - (a2/a1)^2,
+    e_c^2 = 1 - (a3/a1)^2; `m` sets v_c^2 = G m / r_s (nfw/leesuto.py:29-164).
+    Requires a1 >= a2 >= a3."""
+
+    kind = _abi.GX_LEESUTO_NFW
+
+    def __init__(self, m, r_s, a1=1.0, a2=1.0, a3=1.0, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m = _p(m, "Msun")
+        self.r_s = _p(r_s, "kpc")
+        self.a1, self.a2, self.a3 = _p(a1), _p(a2), _p(a3)
+        if self.a1 < self.a2 or self.a2 < self.a3:
+            raise ValueError(f"a1 {self.a1} >= a2 {self.a2} >= a3 {self.a3} is required")
+        if not (self.a3 > 0 and self.r_s > 0):
+            raise ValueError("axes and r_s must be positive")
+
+    def _params(self):
+        return (G * self.m, self.r_s, self.a1, self.a2, self.a3)
+
+    def __repr__(self):
+        return (f"LeeSutoTriaxialNFWPotential(m={self.m!r}, r_s={self.r_s!r}, a1={self.a1!r}, "
+                f"a2={self.a2!r}, a3={self.a3!r})")
+
+
+class Vogelsberger08TriaxialNFWPotential(AbstractSinglePotential):
+    """NFW potential of the radius r~ = (r_a + r) r_e / (r_a + r_e), which is
+    ellipsoidal (r_e^2 = x^2 + y^2/q1^2 + z^2/q2^2, q2^2 = 3 - q1^2) inside
+    r_a = a_r r_s and spherical outside: Phi = -G m log(1 + r~/r_s) / r~
+    (nfw/vogelsburger08.py:27-110)."""
+
+    kind = _abi.GX_VOGELSBERGER_NFW
+
+    def __init__(self, m, r_s, q1=1.0, a_r=1.0, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m = _p(m, "Msun")
+        self.r_s = _p(r_s, "kpc")
+        self.q1, self.a_r = _p(q1), _p(a_r)
+        if not (0 < self.q1 * self.q1 < 3 and self.r_s > 0):
+            raise ValueError("need 0 < q1^2 < 3 and r_s > 0")
+
+    def _params(self):
+        return (G * self.m, self.r_s, self.q1, self.a_r)
+
+    def __repr__(self):
+        return (f"Vogelsberger08TriaxialNFWPotential(m={self.m!r}, r_s={self.r_s!r}, "
+                f"q1={self.q1!r}, a_r={self.a_r!r})")

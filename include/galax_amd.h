@@ -57,6 +57,16 @@ extern "C" {
  *   GX_KUZMIN          p0 = G*m_tot, p1 = r_s (|z| in Phi)   kuzmin.py:60-85
  *   GX_SATOH           p0 = G*m_tot, p1 = a, p2 = b          satoh.py:50-69
  *   GX_STONE           p0 = G*m_tot, p1 = r_c, p2 = r_h      stoneostriker15.py:90-147
+ *   GX_TRIAXIAL_HERNQUIST p0 = G*m_tot, p1 = r_s, p2 = q1, p3 = q2
+ *                      (Phi of r' = sqrt(x^2 + (y/q1)^2 + (z/q2)^2))
+ *                                                            hernquist.py:89-182
+ *   GX_TRIAXIAL_NFW    p0 = G*m, p1 = r_s, p2 = q1, p3 = q2 (density axis
+ *                      ratios), p4 = Gauss-Legendre order (1..128; the
+ *                      reference's default is 50)            nfw/triaxial.py:29-230
+ *   GX_LEESUTO_NFW     p0 = G*m, p1 = r_s, p2..p4 = a1 >= a2 >= a3
+ *                                                            nfw/leesuto.py:29-164
+ *   GX_VOGELSBERGER_NFW p0 = G*m, p1 = r_s, p2 = q1 (q2^2 = 3 - q1^2),
+ *                      p3 = a_r                              nfw/vogelsburger08.py:27-110
  */
 enum gx_kind {
   GX_NULL = 0,
@@ -75,7 +85,11 @@ enum gx_kind {
   GX_BURKERT = 13,
   GX_KUZMIN = 14,
   GX_SATOH = 15,
-  GX_STONE = 16
+  GX_STONE = 16,
+  GX_TRIAXIAL_HERNQUIST = 17,
+  GX_TRIAXIAL_NFW = 18,
+  GX_LEESUTO_NFW = 19,
+  GX_VOGELSBERGER_NFW = 20
 };
 
 /* Where the component sits (potential/_src/xfm/translate.py:29-199 —
@@ -122,6 +136,7 @@ typedef struct gx_pot gx_pot; /* opaque handle; lives on one device */
  * Host arrays; copied.  At most GX_MAX_COMPONENTS components. */
 #define GX_MAX_COMPONENTS 256
 #define GX_MAX_TRACKS 8
+#define GX_MAX_GL_ORDER 128 /* GX_TRIAXIAL_NFW quadrature order limit */
 int gx_potential_create(const gx_component* comps, int32_t ncomp,
                         const gx_track* tracks, int32_t ntracks,
                         gx_pot** out);

@@ -38,6 +38,8 @@ def lib() -> ctypes.CDLL:
         L = ctypes.CDLL(str(LIB_PATH))
         L.orc_eval.restype = ctypes.c_int
         L.orc_eval.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int64, _vp]
+        L.orc_gauss_legendre.restype = ctypes.c_int
+        L.orc_gauss_legendre.argtypes = [ctypes.c_int, _vp, _vp]
         L.orc_tableau.restype = ctypes.c_int
         L.orc_tableau.argtypes = [ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_int)]
         L.orc_rk_step.restype = ctypes.c_int
@@ -107,6 +109,15 @@ def evaluate(pot, what: str, xyz, t=0.0) -> np.ndarray:
     if code == 2:
         return out.reshape(*batch, 3, 3)
     return out.reshape(*batch, 3)
+
+
+def gauss_legendre(n: int):
+    """The oracle's Gauss-Legendre rule on [0, 1]: nodes and weights."""
+    s = np.zeros(n)
+    w = np.zeros(n)
+    if lib().orc_gauss_legendre(int(n), s.ctypes.data, w.ctypes.data) != 0:
+        raise ValueError(f"bad order {n}")
+    return s, w
 
 
 def tableau(method: int):

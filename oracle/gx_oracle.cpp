@@ -16,6 +16,7 @@
 //       NFW             builtin/nfw/base.py:327-338
 //       Plummer         builtin/plummer.py:123-132
 //       Monari bar      builtin/monari2016.py:110-138
+//       and the further models named at each case of phi_model below,
 //     with r = sqrt(x^2+y^2+z^2 + tiny) (potential/_src/utils.py:45-126);
 //   * the composite sum of component gradients (base_multi.py:39-82);
 //   * translation Phi(x - Delta(t)) (xfm/translate.py:100-110);
@@ -38,6 +39,7 @@
 #include "../include/galax_amd.h"
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -284,6 +286,53 @@ J<O> cube(const J<O>& a) {  // x**3
 }
 
 // ------------------------------------------------------- potential models
+// Gauss-Legendre rule of order n mapped to [0, 1] (nodes s_k, weights w_k):
+// the roots of P_n by Newton's method on the three-term recurrence, from
+// the asymptotic guesses cos(pi (k - 1/4) / (n + 1/2)); weights
+// 2 / ((1 - x^2) P_n'(x)^2).  The reference's GaussLegendreIntegrator
+// (potential/_src/utils.py:131-162) takes numpy's leggauss and maps it the
+// same way.  Cached per order.
+const std::vector<double>& gauss_legendre01(int n) {
+  static std::vector<double> cache[GX_MAX_GL_ORDER + 1];
+  static std::once_flag once[GX_MAX_GL_ORDER + 1];
+  std::call_once(once[n], [n] {
+    std::vector<double> sw(2 * n);
+    for (int k = 1; k <= n; ++k) {
+      double x = std::cos(M_PI * (k - 0.25) / (n + 0.5)), dp = 1.0;
+      for (int it = 0; it < 100; ++it) {
+        double p0 = 1.0, p1 = x;
+        for (int j = 2; j <= n; ++j) {
+          const double p2 = ((2.0 * j - 1.0) * x * p1 - (j - 1.0) * p0) / j;
+          p0 = p1;
+          p1 = p2;
+        }
+        if (n == 1) p0 = 1.0;
+        dp = n * (x * p1 - p0) / ((x - 1.0) * (x + 1.0));
+        const double dx = p1 / dp;
+        x -= dx;
+        if (std::fabs(dx) < 1e-16) {
+          // refresh the derivative at the converged root
+          p0 = 1.0;
+          p1 = x;
+          for (int j = 2; j <= n; ++j) {
+            const double p2 = ((2.0 * j - 1.0) * x * p1 - (j - 1.0) * p0) / j;
+            p0 = p1;
+            p1 = p2;
+          }
+          if (n == 1) p0 = 1.0;
+          dp = n * (x * p1 - p0) / ((x - 1.0) * (x + 1.0));
+          break;
+        }
+      }
+      const double w = 2.0 / ((1.0 - x) * (1.0 + x) * dp * dp);
+      sw[2 * (k - 1)] = 0.5 * (x + 1.0);
+      sw[2 * (k - 1) + 1] = 0.5 * w;
+    }
+    cache[n] = std::move(sw);
+  });
+  return cache[n];
+}
+
 template <int O>
 J<O> r_spherical(const J<O>& x, const J<O>& y, const J<O>& z) {
   return jsqrt(sq(x) + sq(y) + sq(z) + kTiny);
@@ -396,6 +445,61 @@ J<O> phi_model(const gx_component& c, const J<O>& x, const J<O>& y,
       J<O> at = (rh * jatan(r / rh) - rc * jatan(r / rc)) / r;
       J<O> lg = 0.5 * jlog((sq(r) + rh * rh) / (sq(r) + rc * rc));
       return (-2.0 * p[0] / (M_PI * (rh - rc))) * (at + lg);
+    }
+    case GX_TRIAXIAL_HERNQUIST: {
+      // -G m / (r' + r_s), r' = sqrt(x^2 + (y/q1)^2 + (z/q2)^2 + tiny)
+      // (builtin/hernquist.py:163-182)
+      J<O> rp = jsqrt(sq(x) + sq(y / p[2]) + sq(z / p[3]) + kTiny);
+      return (-p[0]) / (rp + p[1]);
+    }
+    case GX_TRIAXIAL_NFW: {
+      // -2 pi G rho0 r_s^2 q1 q2 sum_k w_k [2 / (1 + xi_k)] /
+      //   sqrt(((q1^2 - 1) s_k^2 + 1)((q2^2 - 1) s_k^2 + 1)),
+      // r_s^2 xi_k^2 = s_k^2 (x^2 + y^2 / (1 + (q1^2 - 1) s_k^2)
+      //                       + z^2 / (1 + (q2^2 - 1) s_k^2)),
+      // rho0 = m / (4 pi r_s^3)  (builtin/nfw/triaxial.py:103-209)
+      const double rs = p[1], q1sq = p[2] * p[2], q2sq = p[3] * p[3];
+      const std::vector<double>& sw = gauss_legendre01((int)p[4]);
+      J<O> sum = jconst<O>(0.0);
+      for (int k = 0; k < (int)p[4]; ++k) {
+        const double s2 = sw[2 * k] * sw[2 * k];
+        const double A1 = 1.0 + (q1sq - 1.0) * s2, A2 = 1.0 + (q2sq - 1.0) * s2;
+        J<O> xi = jsqrt(s2 * (sq(x) + sq(y) / A1 + sq(z) / A2)) / rs;
+        sum = sum + (sw[2 * k + 1] / std::sqrt(A1 * A2)) * (2.0 / (1.0 + xi));
+      }
+      // 2 pi G rho0 r_s^2 = G m / (2 r_s)
+      return (-p[0] / (2.0 * rs) * p[2] * p[3]) * sum;
+    }
+    case GX_LEESUTO_NFW: {
+      // phi0 [F1 + (e_b^2 + e_c^2)/2 F2 + (e_b^2 sin^2(th) sin^2(ph)
+      //   + e_c^2 cos^2(th))/2 F3] (builtin/nfw/leesuto.py:124-164)
+      const double rs = p[1], a1 = p[2], a2 = p[3], a3 = p[4];
+      const double LOG2 = std::log(2.0);
+      const double vc2 = p[0] / rs;
+      const double eb2 = 1.0 - (a2 / a1) * (a2 / a1), ec2 = 1.0 - (a3 / a1) * (a3 / a1);
+      const double phi0 = vc2 / (LOG2 - 0.5 + (LOG2 - 0.75) * (eb2 + ec2));
+      J<O> r = r_spherical(x, y, z);
+      J<O> s = r / rs;
+      if (s.v == 0.0) return jconst<O>(phi0);
+      J<O> L = jlog1p(s);
+      J<O> u2 = sq(s);
+      J<O> um3 = 1.0 / cube(s);
+      J<O> costh2 = sq(z) / sq(r);
+      J<O> sinsin2 = sq(y) / sq(r);
+      J<O> F1 = -(L / s);
+      J<O> F2 = -1.0 / 3.0 + (2.0 * u2 - 3.0 * s + 6.0) / (6.0 * u2) + (1.0 / s - um3) * L;
+      J<O> F3 = (u2 - 3.0 * s - 6.0) / (2.0 * u2 * (1.0 + s)) + 3.0 * um3 * L;
+      return phi0 * (F1 + (0.5 * (eb2 + ec2)) * F2 + 0.5 * (eb2 * sinsin2 + ec2 * costh2) * F3);
+    }
+    case GX_VOGELSBERGER_NFW: {
+      // -G m log1p(r~ / r_s) / r~, r~ = (r_a + r) r_e / (r_a + r_e),
+      // r_e^2 = x^2 + y^2/q1^2 + z^2/q2^2, q2^2 = 3 - q1^2, r_a = a_r r_s
+      // (builtin/nfw/vogelsburger08.py:60-110)
+      const double rs = p[1], q1sq = p[2] * p[2], q2sq = 3.0 - q1sq, ra = p[3] * rs;
+      J<O> re = jsqrt(sq(x) + sq(y) / q1sq + sq(z) / q2sq + kTiny);
+      J<O> r = r_spherical(x, y, z);
+      J<O> rt = (ra + r) * re / (ra + re);
+      return (-p[0]) * jlog1p(rt / rs) / rt;
     }
     default:
       return jconst<O>(0.0);
@@ -990,6 +1094,17 @@ int orc_eval(const gx_component* comps, int ncomp, const gx_track* tracks,
 }
 
 // Tableau export: fills c[s], a[s*s] (row-major), b[s], be[s]; returns s.
+// Gauss-Legendre rule on [0, 1] of order n (1..GX_MAX_GL_ORDER): s, w of n.
+int orc_gauss_legendre(int n, double* s, double* w) {
+  if (n < 1 || n > GX_MAX_GL_ORDER || !s || !w) return GX_EINVAL;
+  const std::vector<double>& sw = gauss_legendre01(n);
+  for (int k = 0; k < n; ++k) {
+    s[k] = sw[2 * k];
+    w[k] = sw[2 * k + 1];
+  }
+  return GX_OK;
+}
+
 int orc_tableau(int method, double* c, double* a, double* b, double* be,
                 int* order) {
   const Tableau& T = tableau(method);

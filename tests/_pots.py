@@ -91,6 +91,10 @@ def zoo():
         "kuzmin": gp.KuzminPotential(m_tot=2e10, r_s=2.5),
         "satoh": gp.SatohPotential(m_tot=5e10, a=3.0, b=0.6),
         "stone": gp.StoneOstriker15Potential(m_tot=5e11, r_c=2.0, r_h=25.0),
+        "triaxial_hernquist": gp.TriaxialHernquistPotential(m_tot=3e11, r_s=4.0, q1=0.9, q2=0.7),
+        "triaxial_nfw": gp.TriaxialNFWPotential(m=5e11, r_s=12.0, q1=0.9, q2=0.8),
+        "leesuto_nfw": gp.LeeSutoTriaxialNFWPotential(m=6e11, r_s=15.0, a1=1.0, a2=0.9, a3=0.8),
+        "vogelsberger_nfw": gp.Vogelsberger08TriaxialNFWPotential(m=5e11, r_s=12.0, q1=0.9, a_r=1.5),
         "offset_nfw": gp.TranslatedPotential(gp.NFWPotential(m=1e11, r_s=3.0), [1.0, -2.0, 0.5]),
         "offset_bar": gp.TranslatedPotential(bar, [0.5, 0.0, 0.0]),
         "track_plummer": gp.TranslatedPotential(gp.PlummerPotential(m_tot=1e9, r_s=0.3), track),

@@ -20,7 +20,8 @@ from _pots import from_spec, ic, solver_from
 @pytest.mark.parametrize("name", ["kepler", "hernquist", "miyamoto_nagai", "nfw", "plummer", "monari_bar", "milky_way",
                                   "isochrone", "jaffe", "logarithmic", "lmj09_logarithmic", "longmurali_bar",
                                   "mn3_exponential", "mn3_sech2", "milky_way_2022", "lm10", "powerlaw_cutoff",
-                                  "bovy_mw_2014", "burkert", "kuzmin", "satoh", "stone"])
+                                  "bovy_mw_2014", "burkert", "kuzmin", "satoh", "stone", "triaxial_hernquist",
+                                  "triaxial_nfw", "leesuto_nfw", "vogelsberger_nfw"])
 def test_potential_known_answers(golden, name):
     e = next(x for x in golden["potentials"] if x["name"] == name)
     pot = from_spec(e["class"], e["params"])
@@ -51,6 +52,72 @@ def test_potential_known_answers(golden, name):
             assert rho == pytest.approx(e["density"], rel=1e-5)
         else:
             assert rho == pytest.approx(e["density"], rel=5e-7)
+
+
+def test_gauss_legendre_matches_numpy():
+    # the rule behind the triaxial NFW's shell integral: numpy's leggauss
+    # mapped to [0, 1], as the reference builds it (potential/_src/utils.py:138-149)
+    for n in (1, 2, 3, 7, 50, 64, 127, 128):
+        s, w = oracle.gauss_legendre(n)
+        x, wx = np.polynomial.legendre.leggauss(n)
+        order = np.argsort(s)
+        np.testing.assert_allclose(s[order], 0.5 * (x + 1), rtol=0, atol=2e-16)
+        # numpy's own weights carry up to ~3e-11 relative error at n = 128
+        # (50-digit values, checked once); ours ~3e-13
+        np.testing.assert_allclose(w[order], 0.5 * wx, rtol=5e-11, atol=0)
+        # the rule integrates s^j exactly for j < 2n
+        for j in range(0, min(2 * n, 40)):
+            assert np.dot(w, s**j) == pytest.approx(1.0 / (j + 1), rel=1e-13, abs=1e-16), (n, j)
+
+
+def test_triaxial_models_reduce_to_spherical():
+    # with unit axis ratios (Vogelsberger: no transition radius) each
+    # triaxial model is its spherical parent
+    pts = np.array([[1.0, 2.0, 3.0], [0.3, -0.1, 0.05], [40.0, 5.0, -7.0]])
+    pairs = [
+        (gp.TriaxialHernquistPotential(m_tot=3e11, r_s=4.0), gp.HernquistPotential(m_tot=3e11, r_s=4.0)),
+        (gp.TriaxialNFWPotential(m=5e11, r_s=12.0), gp.NFWPotential(m=5e11, r_s=12.0)),
+        # r~ = (r_a + r) r_e / (r_a + r_e) is r itself at r_a = 0, whatever q1
+        (gp.Vogelsberger08TriaxialNFWPotential(m=5e11, r_s=12.0, q1=0.8, a_r=0.0), gp.NFWPotential(m=5e11, r_s=12.0)),
+    ]
+    for tri, sph in pairs:
+        for what in ("potential", "gradient", "hessian"):
+            np.testing.assert_allclose(oracle.evaluate(tri, what, pts), oracle.evaluate(sph, what, pts),
+                                       rtol=1e-12, atol=1e-18, err_msg=f"{tri!r} {what}")
+    # Lee & Suto's normalisation: v_c(r_s)^2 = G m / r_s for the spherical case
+    ls = gp.LeeSutoTriaxialNFWPotential(m=6e11, r_s=15.0)
+    nfw = gp.NFWPotential(m=6e11, r_s=15.0)
+    scale = 1.0 / (math.log(2.0) - 0.5)
+    np.testing.assert_allclose(oracle.evaluate(ls, "potential", pts), scale * oracle.evaluate(nfw, "potential", pts),
+                               rtol=1e-12)
+
+
+def test_leesuto_is_the_first_order_expansion_of_the_exact_triaxial_nfw():
+    """Lee & Suto's F2 and F3 terms (unpinned by the reference's test, whose
+    axes are equal) are the first-order terms in e_b^2, e_c^2 of the exact
+    shell-integral potential of the triaxial NFW density with q^2 = 1 - e^2.
+    Phi_LS / phi0 and Phi_exact r_s / (G m) share their e = 0 value, and their
+    derivatives in e^2 must agree to first order."""
+    m, rs, eps = 1e12, 2.0, 1e-4
+    for x in ([1.0, 2.0, 3.0], [0.3, 0.1, 2.0], [5.0, -1.0, 0.5], [0.2, 3.0, 0.1]):
+        x = np.array(x)
+        # a1 >= a2 >= a3 means e_b^2 <= e_c^2; two independent directions
+        for eb2, ec2 in ((0.0, eps), (0.5 * eps, eps), (eps, eps)):
+            def ls_bracket(e_b2, e_c2):
+                pot = gp.LeeSutoTriaxialNFWPotential(m=m, r_s=rs, a1=1.0, a2=math.sqrt(1 - e_b2),
+                                                     a3=math.sqrt(1 - e_c2))
+                phi0 = (G * m / rs) / (math.log(2.0) - 0.5 + (math.log(2.0) - 0.75) * (e_b2 + e_c2))
+                return oracle.evaluate(pot, "potential", x) / phi0
+
+            def exact(e_b2, e_c2):
+                pot = gp.TriaxialNFWPotential(m=m, r_s=rs, q1=math.sqrt(1 - e_b2), q2=math.sqrt(1 - e_c2),
+                                              integration_order=128)
+                return oracle.evaluate(pot, "potential", x) * rs / (G * m)
+
+            assert ls_bracket(0, 0) == pytest.approx(exact(0, 0), rel=1e-12)
+            d_ls = (ls_bracket(eb2, ec2) - ls_bracket(0, 0)) / eps
+            d_ex = (exact(eb2, ec2) - exact(0, 0)) / eps
+            assert d_ls == pytest.approx(d_ex, rel=2e-4), (x, eb2, ec2)
 
 
 def test_extra_known_answers(golden):
