@@ -64,7 +64,10 @@ def test_field_parity(name):
     H_o = oracle.evaluate(pot, "hessian", x, t)
     # the bar's Phi changes sign, so compare against the field's scale
     assert np.max(np.abs(phi - phi_o)) < 1e-13 * np.max(np.abs(phi_o))
-    assert np.max(_rel(g, g_o)) < 1e-12
+    # Long & Murali's log((x' - a + T-) / (x' + a + T+)) cancels in
+    # x' - a + T- for x' << -a, as the reference writes it: both sides carry
+    # that round-off, amplified by |x' - a| / (x' - a + T-) (~1e4 here)
+    assert np.max(_rel(g, g_o)) < (1e-11 if name == "longmurali" else 1e-12)
     np.testing.assert_array_equal(a, -g)
     hn = np.linalg.norm(H_o.reshape(n, 9), axis=1)
     assert np.max(np.linalg.norm((H - H_o).reshape(n, 9), axis=1) / hn) < 1e-11
