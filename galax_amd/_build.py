@@ -10,7 +10,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB_PATH = PKG / "libgalaxhip.so"
-SOURCES = ("gx_lib.hip", "gx_integrate.hip", "gx_adjoint.hip", "gx_mlp.hip", "gx_nbody.hip")
+SOURCES = ("gx_lib.hip", "gx_integrate.hip", "gx_shared.hip", "gx_adjoint.hip", "gx_mlp.hip", "gx_nbody.hip")
 ARCH = os.environ.get("GALAX_AMD_ARCH", "gfx950")
 
 # Packed FP32 instructions are disabled for every HIP translation unit.

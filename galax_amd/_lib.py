@@ -42,6 +42,10 @@ SIGNATURES = {
         ctypes.c_int,
         [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     ),
+    "gx_integrate_shared_workspace_bytes": (_i64, [_i64]),
+    "gx_integrate_shared": (
+        ctypes.c_int, [_vp, _vp, _vp, _f64, _f64, _vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    ),
     "gx_set_launch_shape": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "gx_device_info": (ctypes.c_int, [ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "gx_math_check": (ctypes.c_int, [_vp, _i64, _vp, _vp]),
@@ -77,6 +81,10 @@ SIGNATURES = {
 
 
 def _declare(lib: ctypes.CDLL) -> None:
+    missing = [name for name in SIGNATURES if not hasattr(lib, name)]
+    if missing:
+        raise GalaxHipError(f"{LIB_PATH} lacks {', '.join(missing)}: it is older than this package; "
+                            "rebuild it with `python -m galax_amd._build` (or __graft_entry__.build())")
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res
@@ -92,8 +100,9 @@ def lib() -> ctypes.CDLL:
                 f"{LIB_PATH} is missing: build it with `python -m galax_amd._build` "
                 "(or __graft_entry__.build())"
             )
-        _lib = ctypes.CDLL(str(LIB_PATH))
-        _declare(_lib)
+        handle = ctypes.CDLL(str(LIB_PATH))
+        _declare(handle)  # every signature, or none: never a half-declared handle
+        _lib = handle
     return _lib
 
 

@@ -175,9 +175,17 @@ class OrbitInterpolant:
 
 class Integrator:
     """galax's legacy Integrator: Dopri8 with PIDController(rtol=1e-7,
-    atol=1e-7) and unbounded max_steps by default."""
+    atol=1e-7) and unbounded max_steps by default.
 
-    def __init__(self, dynamics_solver=None, *, diffeq_kw=None):
+    As in galax (legacy/integrator.py:269-355 and 449-527), a batch with
+    scalar start and end times is solved as one system with a shared step
+    sequence, and a batch with per-orbit times as independent orbits.
+    `shared_steps=False` gives every orbit its own steps in both cases (the
+    fast path for large batches of independent orbits); fixed-step runs are
+    identical either way."""
+
+    def __init__(self, dynamics_solver=None, *, diffeq_kw=None, shared_steps: bool | None = None):
+        self.shared_steps = shared_steps
         if dynamics_solver is None:
             dynamics_solver = OrbitSolver(
                 solver=Dopri8(scan_kind="bounded"),
@@ -188,6 +196,16 @@ class Integrator:
         kw = {"max_steps": None}
         kw.update(diffeq_kw or {})
         self.diffeq_kw = kw
+
+    def _shared(self, w, t0, t1) -> bool:
+        """galax's rule: scalar times couple the batch.  A single orbit, or a
+        fixed step, gives the same steps either way, so those take the
+        independent path."""
+        if self.shared_steps is False:
+            return False
+        scalar = np.ndim(t0) == 0 and np.ndim(t1) == 0
+        adaptive = self.dynamics_solver.stepsize_controller.__class__.__name__ == "PIDController"
+        return scalar and adaptive and int(np.shape(w)[0]) > 1
 
     def _run(self, pot, w, t0, t1, ts=None):
         kw = self.diffeq_kw
@@ -200,6 +218,7 @@ class Integrator:
             solver=self.dynamics_solver,
             dt0=kw.get("dt0"),
             max_steps=kw.get("max_steps", "default"),
+            shared=self._shared(w, t0, t1) and (ts is None or np.ndim(ts) == 1),
         )
 
     def __call__(self, field, w0, t0, t1, *, saveat=None, dense=False):
@@ -227,10 +246,11 @@ def evaluate_orbit(pot, w0, t, *, integrator: Integrator | None = None, dense: b
     """Integrate orbits in `pot`, returning them at the times `t`.
 
     As in galax (legacy/funcs.py:42-213): a PhaseSpaceCoordinate w0 is first
-    integrated from w0.t to t[0]; then every orbit is integrated from t[0] to
-    t[-1] and saved at each t.  Each orbit takes its own adaptive step
-    sequence (galax couples a batch with scalar times into one shared step
-    sequence; fixed-step runs are identical either way)."""
+    integrated from w0.t to t[0] (per-orbit times: independent orbits); then
+    the batch is integrated from t[0] to t[-1] and saved at each t, as one
+    system with a shared adaptive step sequence, as galax does (scalar
+    times).  `Integrator(shared_steps=False)` gives each orbit its own steps
+    instead."""
     integrator = integrator or default_integrator
     t = np.atleast_1d(np.asarray(ustrip(t, "Myr"), dtype=np.float64))
     w, batch, tw0 = parse_w0(w0)

@@ -116,15 +116,46 @@ class OrbitSolver:
         return replace(self, **kw)
 
     def solve(self, field, qp, t0, t1, *, saveat=None, dt0=None, max_steps="default"):
-        """OrbitSolver.solve (orbit/solver.py:121-141) for an NBodyField: the
-        bodies integrate as one system; returns an NBodySolution with
-        ys = (q, p) of shape (T, N, 3)."""
-        from ._nbody import NBodyField, solve_nbody
+        """OrbitSolver.solve (orbit/solver.py:121-141, 774-1039).
+
+        For an NBodyField the bodies integrate as one system.  For a
+        HamiltonianField (or a potential) over a batch of orbits, scalar t0
+        and t1 solve the batch as one ODE with a shared step sequence, as
+        diffeqsolve does over a (B, 6) state; per-orbit t0/t1 give every
+        orbit its own steps (the vectorized path, orbit/solver.py:854-1039).
+        Returns a solution with ts (T,) and ys = (q, p) of shape (T, B, 3)
+        (the saved times first, as diffrax returns them); without `saveat`
+        only t1 is saved."""
+        from ._nbody import NBodyField, NBodySolution, solve_nbody
+        from .fields import HamiltonianField
 
         if isinstance(field, NBodyField):
             return solve_nbody(field, qp, t0, t1, saveat=saveat, solver=self, dt0=dt0, max_steps=max_steps)
-        raise TypeError(f"OrbitSolver.solve supports NBodyField here, not {type(field).__name__}; "
-                        "use integrate_orbits / evaluate_orbit for potentials")
+        pot = field.potential if isinstance(field, HamiltonianField) else field
+        if not hasattr(pot, "_handle"):
+            raise TypeError(f"OrbitSolver.solve needs a potential, HamiltonianField or NBodyField, "
+                            f"not {type(field).__name__}")
+        from ..units import ustrip
+
+        q, p = qp
+        q = np.asarray(ustrip(q, "kpc"), dtype=np.float64).reshape(-1, 3)
+        p = np.asarray(ustrip(p, "kpc/Myr"), dtype=np.float64).reshape(-1, 3)
+        t0v = np.asarray(ustrip(t0, "Myr"), dtype=np.float64)
+        t1v = np.asarray(ustrip(t1, "Myr"), dtype=np.float64)
+        shared = t0v.ndim == 0 and t1v.ndim == 0
+        ts = None if saveat is None else np.atleast_1d(np.asarray(ustrip(saveat, "Myr"), dtype=np.float64))
+        if ts is None and shared:
+            ts = np.atleast_1d(t1v)
+        res = integrate_orbits(pot, np.concatenate([q, p], 1), t0v, t1v, ts=ts, solver=self, dt0=dt0,
+                               max_steps=max_steps, shared=shared)
+        out = res.out.cpu().numpy()  # (B, T, 6)
+        ys = (np.transpose(out[..., :3], (1, 0, 2)), np.transpose(out[..., 3:], (1, 0, 2)))
+        st = res.status.cpu().numpy()
+        ns = res.nsteps.cpu().numpy()
+        na = res.naccept.cpu().numpy()
+        if shared:
+            return NBodySolution(ts=ts, ys=ys, status=int(st[0]), nsteps=int(ns[0]), naccept=int(na[0]))
+        return NBodySolution(ts=ts if ts is not None else t1v, ys=ys, status=st, nsteps=ns, naccept=na)
 
 
 @dataclass
@@ -168,12 +199,16 @@ def integrate_orbits(
     out=None,
     workspace=None,
     stream=None,
+    shared: bool = False,
 ) -> IntegrationResult:
-    """Integrate n independent orbits on the GPU.
+    """Integrate n orbits on the GPU.
 
     w0: (n, 6); t0, t1: scalars or (n,); ts: (T,) shared save times, (n, T)
     per-orbit save times, or None (with `T`: per-orbit linspace(t0, t1, T);
-    without: the final state only).  Returns device tensors."""
+    without: the final state only).  Every orbit takes its own step sequence
+    unless `shared`: then the batch is one system with one step sequence
+    (scalar t0, t1 and shared or no save times), as galax's Integrator
+    solves a batch with scalar times.  Returns device tensors."""
     torch = _lib.require_gpu()
     solver = solver or OrbitSolver()
     cfg = solver.cfg(dt0=dt0, max_steps=max_steps)
@@ -181,6 +216,8 @@ def integrate_orbits(
     if w0d.ndim != 2 or w0d.shape[1] != 6:
         raise ValueError(f"w0 must have shape (n, 6), got {tuple(w0d.shape)}")
     n = int(w0d.shape[0])
+    if shared:
+        return _integrate_shared(torch, pot, cfg, w0d, n, t0, t1, ts, T, out, workspace, stream)
     t0d = _dev(torch, t0, torch.float64).reshape(-1)
     t1d = _dev(torch, t1, torch.float64).reshape(-1)
     if t0d.numel() == 1 and t1d.numel() == 1:
@@ -244,4 +281,47 @@ def integrate_orbits(
         stream if stream is not None else _lib.stream_ptr(),
     )
     _lib.check(rc, "gx_integrate")
+    return IntegrationResult(out=out, status=status, nsteps=nsteps, naccept=naccept)
+
+
+def _integrate_shared(torch, pot, cfg, w0d, n, t0, t1, ts, T, out, workspace, stream) -> IntegrationResult:
+    t0v = np.asarray(t0.cpu() if isinstance(t0, torch.Tensor) else t0, dtype=np.float64)
+    t1v = np.asarray(t1.cpu() if isinstance(t1, torch.Tensor) else t1, dtype=np.float64)
+    if t0v.size != 1 or t1v.size != 1:
+        raise ValueError("a shared step sequence needs scalar t0 and t1")
+    t0f, t1f = float(t0v.reshape(())), float(t1v.reshape(()))
+    if ts is None:
+        ts = np.linspace(t0f, t1f, T) if T is not None else np.array([t1f])
+    tsd = _dev(torch, ts, torch.float64)
+    if tsd.ndim != 1:
+        raise ValueError("a shared step sequence needs shared save times ts of shape (T,)")
+    T = int(tsd.shape[0])
+    if T < 1:
+        raise ValueError("need at least one save time")
+    if out is None:
+        out = torch.empty((n, T, 6), dtype=torch.float64, device="cuda")
+    status = torch.empty((n,), dtype=torch.int32, device="cuda")
+    nsteps = torch.empty((n,), dtype=torch.int64, device="cuda")
+    naccept = torch.empty((n,), dtype=torch.int64, device="cuda")
+    lib = _lib.lib()
+    nbytes = int(lib.gx_integrate_shared_workspace_bytes(n))
+    if workspace is None or workspace.numel() * workspace.element_size() < nbytes:
+        workspace = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device="cuda")
+    rc = lib.gx_integrate_shared(
+        pot._handle(),
+        __import__("ctypes").addressof(cfg),
+        _lib.ptr(w0d),
+        t0f,
+        t1f,
+        _lib.ptr(tsd),
+        T,
+        n,
+        _lib.ptr(out),
+        _lib.ptr(status),
+        _lib.ptr(nsteps),
+        _lib.ptr(naccept),
+        _lib.ptr(workspace),
+        stream if stream is not None else _lib.stream_ptr(),
+    )
+    _lib.check(rc, "gx_integrate_shared")
     return IntegrationResult(out=out, status=status, nsteps=nsteps, naccept=naccept)

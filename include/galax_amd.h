@@ -233,6 +233,28 @@ int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
                  double* out, int32_t* status, int64_t* nsteps,
                  int64_t* naccept, void* workspace, void* stream);
 
+/* Integrate n orbits as ONE system with a shared step sequence: galax's
+ * Integrator with scalar start and end times solves a batch (B, 6) as one
+ * ODE (legacy/integrator.py:269-355, orbit/solver.py:774-812; the second
+ * stage of gd.evaluate_orbit, legacy/funcs.py:42-213) -- one step size, one
+ * accept/reject decision per step from the RMS error over all 6n components,
+ * the automatic first step from norms over the whole batch.  From t0 to t1,
+ * saving at ts[T] (ordered in the direction of integration).  Under
+ * GX_CONSTANT the result equals gx_integrate's.
+ *
+ * status[n], nsteps[n], naccept[n] (optional) receive the batch's values for
+ * every orbit.  `workspace`: gx_integrate_shared_workspace_bytes(n) bytes of
+ * device memory (256-byte aligned) that no concurrent call uses.
+ * Synchronous: the host launches one kernel per attempted step, in batches,
+ * and returns when the solve has ended (it reads a 4-byte flag after each
+ * batch); the results are then on `stream`. */
+int64_t gx_integrate_shared_workspace_bytes(int64_t n);
+int gx_integrate_shared(const gx_pot* pot, const gx_solver_cfg* cfg,
+                        const double* w0, double t0, double t1,
+                        const double* ts, int32_t T, int64_t n, double* out,
+                        int32_t* status, int64_t* nsteps, int64_t* naccept,
+                        void* workspace, void* stream);
+
 /* ------------------------------------------------------------- adjoint */
 /* Discrete adjoint of fixed-step (GX_CONSTANT) Dopri8/Dopri5 integration of
  * n independent orbits, as gx_integrate would run them with the same

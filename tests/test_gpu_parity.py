@@ -268,7 +268,8 @@ def test_evaluate_orbit_two_stage():
     solver = integ.dynamics_solver
     w = w0.w()
     s1, *_ = oracle.integrate(pot, solver, w, np.array([-100.0, -150.0]), 0.0, max_steps=None)
-    s2, *_ = oracle.integrate(pot, solver, s1[:, 0], 0.0, 1000.0, ts=ts, max_steps=None)
+    # galax couples the second stage (scalar times) into one step sequence
+    s2, *_ = oracle.integrate(pot, solver, s1[:, 0], 0.0, 1000.0, ts=ts, max_steps=None, shared=True)
     assert np.max(_rel(orbit.q, s2[..., :3])) < 1e-8
     # a single time
     o1 = gd.evaluate_orbit(pot, w0, 500.0)

@@ -35,6 +35,22 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_stale_library_is_refused_whole(tmp_path, monkeypatch):
+    """A library that lacks an entry point of this package is refused before
+    any signature is declared, and the loader keeps no handle: a later call
+    must not reach a function with default (32-bit) argument conversion."""
+    src = tmp_path / "stale.c"
+    src.write_text("int gx_potential_create(void){return 0;}\n")
+    so = tmp_path / "libstale.so"
+    subprocess.run(["gcc", "-shared", "-fPIC", "-o", str(so), str(src)], check=True)
+    monkeypatch.setattr(_lib, "LIB_PATH", so)
+    monkeypatch.setattr(_lib, "_lib", None)
+    for _ in range(2):
+        with pytest.raises(_lib.GalaxHipError, match="older than this package"):
+            _lib.lib()
+        assert _lib._lib is None
+
+
 def test_ctypes_signatures_match_header():
     """Every declared entry point has a ctypes signature with the same number
     of arguments (a mismatch would pass garbage through the ABI)."""

@@ -1,0 +1,158 @@
+"""One step sequence shared by a batch (galax's scalar-time Integrator path,
+legacy/integrator.py:269-355; orbit/solver.py:774-812): the GPU's shared
+solve against the reference's eight-decimal doctests and against the
+oracle's shared solve on the same seeded inputs."""
+
+import numpy as np
+import pytest
+
+import oracle
+from galax_amd import _abi, _lib
+import galax_amd.dynamics as gd
+import galax_amd.potential as gp
+from galax_amd import workloads
+
+from _pots import from_spec, ic, solver_from, zoo
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b, axis=-1) / np.maximum(np.linalg.norm(b, axis=-1), 1e-300)
+
+
+# ------------------------------------------------------------------ CPU
+def test_shared_routing_follows_galax():
+    """Scalar times couple the batch under an adaptive controller; per-orbit
+    times, a single orbit, fixed steps or shared_steps=False do not."""
+    integ = gd.Integrator()
+    w2 = np.zeros((2, 6))
+    assert integ._shared(w2, 0.0, 100.0)
+    assert not integ._shared(w2, np.zeros(2), np.full(2, 100.0))
+    assert not integ._shared(np.zeros((1, 6)), 0.0, 100.0)
+    assert not gd.Integrator(shared_steps=False)._shared(w2, 0.0, 100.0)
+    fixed = gd.Integrator(gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.ConstantStepSize()),
+                          diffeq_kw={"dt0": 1.0})
+    assert not fixed._shared(w2, 0.0, 100.0)
+
+
+def test_shared_entry_point_exported():
+    import ctypes
+
+    L = ctypes.CDLL(str(_lib.LIB_PATH))
+    for name in ("gx_integrate_shared", "gx_integrate_shared_workspace_bytes"):
+        assert hasattr(L, name), name
+    ws = _lib.lib().gx_integrate_shared_workspace_bytes
+    # control record + ticket + two partials per 256-orbit workgroup + two
+    # state buffers and the start accelerations
+    assert ws(1000) == 512 + 16 * 4 + 8 * 15 * 1000
+    assert ws(-1) == 0
+
+
+# ------------------------------------------------------------------ GPU
+@pytest.mark.gpu
+@pytest.mark.parametrize("key", ["solve_batch_shared_pid1e8", "run_shared_pid1e8"])
+def test_shared_doctests_on_gpu(golden, key):
+    e = golden["integrator"][key]
+    pot = from_spec(e["potential"]["class"], e["potential"]["params"])
+    w0, t1 = ic(e)
+    solver = solver_from(e["solver"])
+    say("shared doctest", key)
+    res = gd.integrate_orbits(pot, w0, e["t0"], float(t1), solver=solver, shared=True)
+    out = res.out.cpu().numpy()
+    assert np.all(res.status.cpu().numpy() == 0)
+    np.testing.assert_allclose(out[:, 0, :3], np.array(e["q"]).reshape(-1, 3), atol=5.5e-9)
+    np.testing.assert_allclose(out[:, 0, 3:], np.array(e["p"]).reshape(-1, 3), atol=5.5e-9)
+    ref, st, ns, na = oracle.integrate(pot, solver, w0, e["t0"], float(t1), shared=True)
+    assert int(res.nsteps.cpu()[0]) == int(ns[0]) and int(res.naccept.cpu()[0]) == int(na[0])
+    np.testing.assert_allclose(out, ref, rtol=0, atol=1e-11)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["mw_dopri8", "generic_dopri5", "frames_dopri8", "backward", "auto_first_step"])
+def test_shared_matches_oracle(case):
+    """Seeded batches through every evaluator family; save times inside
+    steps and at the end."""
+    rng = np.random.default_rng(21)
+    if case == "mw_dopri8":
+        pot, meth, n, t0, t1, dt0 = gp.MilkyWayPotential(), gd.Dopri8(), 300, 0.0, 800.0, 1.0
+    elif case == "generic_dopri5":
+        pot, meth, n, t0, t1, dt0 = zoo()["lmj09"], gd.Dopri5(), 200, 0.0, 500.0, 0.5
+    elif case == "frames_dopri8":
+        pot, meth, n, t0, t1, dt0 = zoo()["mw_bar_subs"], gd.Dopri8(), 130, 0.0, 400.0, 1.0
+    elif case == "backward":
+        pot, meth, n, t0, t1, dt0 = gp.MilkyWayPotential(), gd.Dopri8(), 257, 300.0, -400.0, 1.0
+    else:
+        pot, meth, n, t0, t1, dt0 = zoo()["triaxial_nfw"], gd.Dopri8(), 64, 0.0, 600.0, None
+    R = rng.uniform(4.0, 15.0, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    x = np.stack([R * np.cos(ph), R * np.sin(ph), rng.normal(0, 0.3, n)], 1)
+    v = np.stack([-0.2 * np.sin(ph), 0.2 * np.cos(ph), np.zeros(n)], 1) + rng.normal(0, 0.02, (n, 3))
+    w0 = np.concatenate([x, v], 1)
+    ts = np.linspace(t0, t1, 13)
+    solver = gd.OrbitSolver(meth, stepsize_controller=gd.PIDController(rtol=1e-8, atol=1e-8), max_steps=None)
+    say("shared vs oracle", case, n)
+    res = gd.integrate_orbits(pot, w0, t0, t1, ts=ts, solver=solver, dt0=dt0, shared=True)
+    out = res.out.cpu().numpy()
+    ref, st, ns, na = oracle.integrate(pot, solver, w0, t0, t1, ts=ts, dt0=dt0, shared=True)
+    assert np.all(res.status.cpu().numpy() == 0) and np.all(st == 0)
+    nst = res.nsteps.cpu().numpy()
+    assert np.all(nst == nst[0])
+    e = np.maximum(_rel(out[..., :3], ref[..., :3]), _rel(out[..., 3:], ref[..., 3:]))
+    print("steps", nst[0], ns[0], "max rel", e.max(), flush=True)
+    if nst[0] == ns[0]:
+        assert e.max() < 1e-9
+    else:
+        # an accept/reject decision within round-off of the threshold flipped
+        assert e.max() < 1e-6
+
+
+@pytest.mark.gpu
+def test_shared_fixed_step_equals_independent():
+    w0, _, _ = workloads.config_uneven(n=700, seed=3)
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.ConstantStepSize(), max_steps=None)
+    ts = np.linspace(0.0, 200.0, 9)
+    say("shared fixed step")
+    a = gd.integrate_orbits(gp.MilkyWayPotential(), w0, 0.0, 200.0, ts=ts, solver=solver, dt0=0.7, shared=True)
+    b = gd.integrate_orbits(gp.MilkyWayPotential(), w0, 0.0, 200.0, ts=ts, solver=solver, dt0=0.7)
+    np.testing.assert_array_equal(a.out.cpu().numpy(), b.out.cpu().numpy())
+    assert np.all(a.nsteps.cpu().numpy() == b.nsteps.cpu().numpy())
+
+
+@pytest.mark.gpu
+def test_shared_status_and_edges():
+    pot = gp.HernquistPotential(m_tot=1e12, r_s=5.0)
+    w0 = np.array([[8.0, 0, 0, 0, 0.22, 0], [9.0, 0, 0, 0, 0.23, 0], [10.0, 0, 0, 0, 0.2, 0]])
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.PIDController(rtol=1e-8, atol=1e-8), max_steps=5)
+    say("shared max_steps")
+    res = gd.integrate_orbits(pot, w0, 0.0, 1000.0, ts=np.array([0.0, 500.0, 1000.0]), solver=solver, shared=True)
+    st = res.status.cpu().numpy()
+    out = res.out.cpu().numpy()
+    assert np.all(st == _abi.GX_MAX_STEPS_REACHED) and np.all(res.nsteps.cpu().numpy() == 5)
+    np.testing.assert_array_equal(out[:, 0], w0)
+    assert np.all(np.isnan(out[:, 2]))
+    ref, rst, rns, _ = oracle.integrate(pot, solver, w0, 0.0, 1000.0, ts=np.array([0.0, 500.0, 1000.0]), shared=True)
+    assert np.all(rst == _abi.GX_MAX_STEPS_REACHED) and rns[0] == 5
+    say("shared t0 == t1")
+    same = gd.integrate_orbits(pot, w0, 5.0, 5.0, ts=np.array([5.0, 5.0]), solver=solver, shared=True)
+    np.testing.assert_array_equal(same.out.cpu().numpy()[:, 1], w0)
+    assert np.all(same.nsteps.cpu().numpy() == 0)
+    with pytest.raises(ValueError):
+        gd.integrate_orbits(pot, w0, np.zeros(3), 10.0, solver=solver, shared=True)
+
+
+@pytest.mark.gpu
+def test_evaluate_orbit_batch_is_shared(golden):
+    """gd.evaluate_orbit over a batch reproduces galax's coupled steps: the
+    reference's batched doctest (orbit/solver.py:602-613) through the
+    two-stage entry point."""
+    e = golden["integrator"]["solve_batch_shared_pid1e8"]
+    pot = from_spec(e["potential"]["class"], e["potential"]["params"])
+    w0, _ = ic(e)
+    integ = gd.Integrator(solver_from(e["solver"]))
+    say("evaluate_orbit shared")
+    orbit = gd.evaluate_orbit(pot, w0, np.array([0.0, 1000.0]), integrator=integ)
+    np.testing.assert_allclose(orbit.q[:, -1], np.array(e["q"]), atol=5.5e-9)
+    np.testing.assert_allclose(orbit.p[:, -1], np.array(e["p"]), atol=5.5e-9)
