@@ -6,10 +6,18 @@
 // its own step sequence.
 //
 // Design (one orbit per lane, persistent launch):
-//   * the grid is sized to the number of resident waves; each lane starts on
-//     orbit `lane`, and when it finishes it takes the next ticket from a
-//     device-wide atomic counter (the work queue), so lanes never idle while
-//     orbits of uneven length remain;
+//   * the grid is sized to the number of resident waves; lanes take orbits
+//     as tickets from a device-wide atomic counter (the work queue), so lanes
+//     never idle while orbits of uneven length remain;
+//   * the head of a cost-ordered queue (the longest orbits) bounds the
+//     launch.  A wave stalls as a whole whenever one of its lanes takes a
+//     ticket (a device-scope atomic, then dependent loads of the queue
+//     entry and the new orbit's state), so in a large grid the first
+//     kHeadLanes lanes integrate tickets 0..kHeadLanes-1 and take no more:
+//     their waves run the bare step loop at top wave priority.  Workgroups
+//     that find themselves on a head workgroup's compute unit (read from
+//     the hardware registers) take no tickets either, so the head waves own
+//     their SIMDs.  The rest of the grid drains the queue;
 //   * the state, the stage accelerations (Runge-Kutta-Nystrom form, 3
 //     doubles per stage), the error norm and the PID controller live in
 //     registers; all arithmetic is FP64;
@@ -51,12 +59,24 @@ struct IArgs {
   long long* naccept;
   unsigned long long* counter;
   double* h0;  // first step per orbit (workspace)
-  long long nlanes;
+  unsigned int* head_loc;  // compute unit of each head workgroup (workspace)
+  long long head_lanes;    // lanes that start on a fixed ticket
+  int head_blocks;
+  int isolate;  // head lanes take one ticket; their neighbours none
 };
 
 constexpr int kBlock = 256;
 constexpr long long kStepCap = 10000000;  // attempted steps per orbit
 constexpr long long kHeadLanes = 4096;     // lanes that start on the queue head
+constexpr int kMaxHeadBlocks = 32;         // head-location slots in the workspace
+
+// Where this wave runs: its XCD and its compute unit within it (the
+// HW_ID fields CU_ID, SH_ID, SE_ID), never 0.  Read-only hardware registers.
+__device__ __forceinline__ unsigned int cu_location() {
+  const unsigned int hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);    // HW_ID
+  const unsigned int xcc = __builtin_amdgcn_s_getreg((15 << 11) | 20);  // XCC_ID
+  return 0x80000000u | ((xcc & 0xFu) << 16) | (hw & 0xFF00u);
+}
 
 // diffrax's automatic first step (Hairer, Norsett & Wanner II.4; final
 // exponent 1/(p+1) for a method of order p) for one orbit in solver
@@ -205,17 +225,49 @@ struct Cold {
   int si[kBlock], st[kBlock], at_dtmin[kBlock];
 };
 
+// True when a head workgroup has recorded `loc` (agent-scope loads: the
+// records are written by other workgroups during this launch).
+__device__ __forceinline__ bool on_head_cu(const IArgs& A, unsigned int loc) {
+  for (int k = 0; k < A.head_blocks; ++k)
+    if (__hip_atomic_load(&A.head_loc[k], __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT) == loc)
+      return true;
+  return false;
+}
+
+// The lane's next ticket, or -1: the queue is empty, or (when isolating
+// the head) the lane is a head lane, or shares a head workgroup's compute
+// unit.
+__device__ __forceinline__ long long next_orbit(const IArgs& A,
+                                                unsigned int loc, bool head) {
+  if (A.isolate && (head || on_head_cu(A, loc))) return -1;
+  const long long idx =
+      A.head_lanes + (long long)atomicAdd(A.counter, 1ULL);
+  return ticket_to_orbit(A, idx);
+}
+
 template <class M, bool PID, class E>
 __global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
   __shared__ Cold C;
+  __shared__ unsigned int s_loc;
   const int L = threadIdx.x;
   const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long orb = ticket_to_orbit(A, lane);
-  bool fresh = true;
+  const bool head = blockIdx.x < (unsigned)A.head_blocks;
+  if (threadIdx.x == 0) {
+    s_loc = cu_location();
+    if (head)
+      __hip_atomic_store(&A.head_loc[blockIdx.x], s_loc, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  const unsigned int loc = s_loc;
   // The first tickets of a cost-ordered queue hold the longest orbits, whose
-  // step latency bounds the launch: their waves issue ahead of the others
-  // on a shared SIMD (wave priority only; the queue is unchanged).
-  if ((long long)blockIdx.x * blockDim.x < kHeadLanes) __builtin_amdgcn_s_setprio(3);
+  // step latency bounds the launch: their waves start on them, issue ahead
+  // of any other wave on their SIMD, and keep the SIMD to themselves once
+  // the workgroups beside them stop taking tickets.
+  long long orb = head ? ticket_to_orbit(A, lane) : next_orbit(A, loc, false);
+  bool fresh = true;
+  if (head) __builtin_amdgcn_s_setprio(3);
 
   double q[3], p[3];
   double tau = 0.0, tnext = 0.0, dir = 1.0;
@@ -411,16 +463,18 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
       if (A.status) A.status[orb] = st;
       if (A.nsteps) A.nsteps[orb] = C.nst[L];
       if (A.naccept) A.naccept[orb] = C.nacc[L];
-      const long long idx =
-          A.nlanes + (long long)atomicAdd(A.counter, 1ULL);
-      orb = ticket_to_orbit(A, idx);
+      orb = next_orbit(A, loc, head);
       fresh = true;
     }
   }
 }
 
-__global__ void k_zero_counter(unsigned long long* c) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ULL;
+// Clear the queue counter and the head-location records (a kernel, not a
+// memset, so that a captured launch sequence replays correctly).
+__global__ void k_reset_queue(unsigned long long* c, unsigned int* loc) {
+  if (blockIdx.x != 0) return;
+  if (threadIdx.x == 0) *c = 0ULL;
+  if (threadIdx.x < kMaxHeadBlocks) loc[threadIdx.x] = 0u;
 }
 
 template <class M, bool PID, class E>
@@ -440,11 +494,20 @@ int launch(const gx_pot* pot, IArgs& A, hipStream_t stream) {
   const long long need = (A.n + block - 1) / block;
   if (blocks > need) blocks = need;
   if (blocks < 1) blocks = 1;
-  A.nlanes = blocks * block;
+  A.head_blocks = (int)(kHeadLanes / block);
+  if (A.head_blocks > kMaxHeadBlocks) A.head_blocks = kMaxHeadBlocks;
+  if (A.head_blocks > blocks) A.head_blocks = (int)blocks;
+  A.head_lanes = (long long)A.head_blocks * block;
+  // Isolation needs a grid whose other workgroups certainly drain the
+  // queue: a compute unit holds at most `occ` of these workgroups, so at
+  // most occ - 1 neighbours per head workgroup retire, and the condition
+  // leaves at least head_blocks workgroups taking tickets.
+  A.isolate = blocks >= (long long)A.head_blocks * (occ + 1) ? 1 : 0;
   long long fblocks = (A.n + 255) / 256;
   const long long fcap = (long long)num_cus(pot->device) * 8;
   if (fblocks > fcap) fblocks = fcap;
-  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, stream, A.counter);
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, stream, A.counter,
+                     A.head_loc);
   hipLaunchKernelGGL((k_first_step<M::order, PID, FULL>), dim3((unsigned)fblocks),
                      dim3(256), 0, stream, pot->dev, A);
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), 0, stream,
@@ -519,6 +582,7 @@ int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
   A.nsteps = reinterpret_cast<long long*>(nsteps);
   A.naccept = reinterpret_cast<long long*>(naccept);
   A.counter = static_cast<unsigned long long*>(workspace);
+  A.head_loc = reinterpret_cast<unsigned int*>(static_cast<char*>(workspace) + 64);
   A.h0 = reinterpret_cast<double*>(static_cast<char*>(workspace) + 256);
 
   hipStream_t s = (hipStream_t)stream;

@@ -18,7 +18,7 @@ from ._mockstream import (
     omega,
     tidal_radius,
 )
-from . import fields
+from . import experimental, fields
 from ._nbody import NBodyField, NBodySolution, solve_nbody
 from ._solver import (
     ConstantStepSize,
@@ -31,6 +31,7 @@ from ._solver import (
 )
 
 __all__ = [
+    "experimental",
     "fields",
     "NBodyField",
     "NBodySolution",

@@ -62,15 +62,21 @@ def test_field_parity(name):
     phi_o = oracle.evaluate(pot, "potential", x, t)
     g_o = oracle.evaluate(pot, "gradient", x, t)
     H_o = oracle.evaluate(pot, "hessian", x, t)
+    # Two models are ill-conditioned as the reference writes them, and both
+    # sides carry that round-off: Long & Murali's log((x' - a + T-) / (x' + a
+    # + T+)) cancels in x' - a + T- for x' << -a (amplification ~1e4 here);
+    # Lee & Suto's F2, F3 cancel as 1/s^3 at small s = r/r_s (the
+    # reference's own float64 Phi is off by 1.7e-13 of its scale at r = 0.35
+    # kpc against 50-digit values, its derivatives by one and two more
+    # powers of 1/s).
+    tol_phi, tol_g, tol_h = {"longmurali": (1e-13, 1e-11, 1e-10),
+                             "leesuto_nfw": (2e-12, 1e-9, 5e-8)}.get(name, (1e-13, 1e-12, 1e-11))
     # the bar's Phi changes sign, so compare against the field's scale
-    assert np.max(np.abs(phi - phi_o)) < 1e-13 * np.max(np.abs(phi_o))
-    # Long & Murali's log((x' - a + T-) / (x' + a + T+)) cancels in
-    # x' - a + T- for x' << -a, as the reference writes it: both sides carry
-    # that round-off, amplified by |x' - a| / (x' - a + T-) (~1e4 here)
-    assert np.max(_rel(g, g_o)) < (1e-11 if name == "longmurali" else 1e-12)
+    assert np.max(np.abs(phi - phi_o)) < tol_phi * np.max(np.abs(phi_o))
+    assert np.max(_rel(g, g_o)) < tol_g
     np.testing.assert_array_equal(a, -g)
     hn = np.linalg.norm(H_o.reshape(n, 9), axis=1)
-    assert np.max(np.linalg.norm((H - H_o).reshape(n, 9), axis=1) / hn) < 1e-11
+    assert np.max(np.linalg.norm((H - H_o).reshape(n, 9), axis=1) / hn) < tol_h
 
 
 def test_field_golden(golden):
@@ -254,6 +260,28 @@ def test_queue_order_does_not_change_results():
     a = gd.integrate_orbits(gp.MilkyWayPotential(), w0, t0, T, T=4, solver=solver)
     b = gd.integrate_orbits(gp.MilkyWayPotential(), w0, t0, T, T=4, solver=solver, order=workloads.cost_order(w0, T))
     np.testing.assert_array_equal(a.out.cpu().numpy(), b.out.cpu().numpy())
+
+
+def test_queue_head_isolation_does_not_change_results():
+    """A grid large enough for the queue-head policy (head lanes take one
+    ticket, their compute-unit neighbours none, the rest drain the queue):
+    every orbit is integrated, bitwise as with another queue order, and as
+    the oracle integrates it."""
+    n = 24576
+    w0, t0, T = workloads.config_uneven(n=n, seed=12)
+    T = np.minimum(T, 150.0)
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.PIDController(rtol=1e-8, atol=1e-10), max_steps=None)
+    say("queue head isolation", n)
+    a = gd.integrate_orbits(gp.MilkyWayPotential(), w0, t0, T, T=3, solver=solver, order=workloads.cost_order(w0, T))
+    b = gd.integrate_orbits(gp.MilkyWayPotential(), w0, t0, T, T=3, solver=solver,
+                            order=np.random.default_rng(3).permutation(n))
+    oa = a.out.cpu().numpy()
+    assert np.all(a.status.cpu().numpy() == 0) and not np.isnan(oa).any()
+    np.testing.assert_array_equal(oa, b.out.cpu().numpy())
+    np.testing.assert_array_equal(a.nsteps.cpu().numpy(), b.nsteps.cpu().numpy())
+    idx = np.random.default_rng(4).choice(n, 96, replace=False)
+    ref, st, ns, na = oracle.integrate(gp.MilkyWayPotential(), solver, w0[idx], t0[idx], T[idx], T=3)
+    check_adaptive(oa[idx], ref, a.nsteps.cpu().numpy()[idx], ns, all_tol=1e-6)
 
 
 def test_evaluate_orbit_two_stage():

@@ -241,6 +241,22 @@ def test_adaptive_doctests(golden, key):
     np.testing.assert_allclose(out[:, 0, 3:], np.array(e["p"]).reshape(-1, 3), atol=tol)
 
 
+def test_experimental_integrate_orbit_docstring(golden):
+    """The experimental API's default solver (Dopri8, PID 1e-7 with dtmin 0.3
+    forced, experimental/integrate.py:28-35) over 10 Myr with 100 saves, most
+    inside steps: eight decimals of its docstring."""
+    from galax_amd.dynamics import experimental as gdx
+
+    e = golden["integrator"]["experimental_integrate_orbit_nfw"]
+    pot = from_spec(e["potential"]["class"], e["potential"]["params"])
+    w0 = np.array([e["q0"] + e["p0"]])
+    out, st, ns, na = oracle.integrate(pot, gdx.default_solver, w0, 0.0, 10.0, ts=np.linspace(0.0, 10.0, 100))
+    assert st[0] == 0
+    for k, row in e["rows"].items():
+        np.testing.assert_allclose(out[0, int(k), :3], row["q"], atol=5.5e-9)
+        np.testing.assert_allclose(out[0, int(k), 3:], row["p"], atol=5.5e-9)
+
+
 @pytest.mark.parametrize("key", ["integrator_default_pid1e7", "readme_mw_default"])
 def test_three_decimal_doctests(golden, key):
     e = golden["integrator"][key]
