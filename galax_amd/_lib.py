@@ -47,6 +47,7 @@ SIGNATURES = {
         ctypes.c_int, [_vp, _vp, _vp, _f64, _f64, _vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     ),
     "gx_set_launch_shape": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "gx_set_launch_fraction": (ctypes.c_int, [_i32, ctypes.POINTER(_i32)]),
     "gx_device_info": (ctypes.c_int, [ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "gx_math_check": (ctypes.c_int, [_vp, _i64, _vp, _vp]),
     "gx_nbody_acceleration": (ctypes.c_int, [_vp, _vp, _vp, _f64, _f64, _i64, _vp, _vp]),

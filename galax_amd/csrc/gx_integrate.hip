@@ -58,7 +58,7 @@ struct IArgs {
   long long* nsteps;
   long long* naccept;
   unsigned long long* counter;
-  double* h0;  // first step per orbit (workspace)
+  double* h0;  // first step per orbit, from the prepass (workspace)
   unsigned int* head_loc;  // compute unit of each head workgroup (workspace)
   long long head_lanes;    // lanes that start on a fixed ticket
   int head_blocks;
@@ -80,8 +80,9 @@ __device__ __forceinline__ unsigned int cu_location() {
 
 // diffrax's automatic first step (Hairer, Norsett & Wanner II.4; final
 // exponent 1/(p+1) for a method of order p) for one orbit in solver
-// variables.  Two field evaluations, one call site.
-template <bool FULL>
+// variables, with the integrator's own field evaluator.  Two field
+// evaluations, one call site.
+template <class E>
 __device__ double initial_step(const DPot& P, double dir, double tau0,
                                const double (&q)[3], const double (&p)[3],
                                double rtol, double atol, double order) {
@@ -103,7 +104,7 @@ __device__ double initial_step(const DPot& P, double dir, double tau0,
       t = tau0 + h0;
     }
     double ax, ay, az;
-    accel<FULL>(P, x, y, z, dir * t, ax, ay, az);
+    E::acc(P, x, y, z, dir * t, ax, ay, az);
     if (pass == 0) {
       a0[0] = ax;
       a0[1] = ay;
@@ -183,10 +184,12 @@ __device__ __forceinline__ long long ticket_to_orbit(const IArgs& A,
   return (o >= 0 && o < A.n) ? o : -1;  // the host validates permutations
 }
 
-// First-step pre-pass: one lane per orbit computes the step the controller
-// starts with (dt0, or diffrax's automatic choice clipped to [dtmin, dtmax])
-// into h0[orbit], so that the persistent kernel carries no extra copy of the
-// field evaluation.
+// First-step prepass for the large evaluators: one lane per orbit computes
+// the step the controller starts with (dt0, or diffrax's automatic choice
+// clipped to [dtmin, dtmax]) into h0[orbit], so that the persistent kernel
+// carries no extra copy of their field evaluation.  (The small Milky Way
+// evaluator computes it in the persistent kernel instead: a separate launch
+// would wait for resident slots behind concurrent integrator launches.)
 template <int ORDER, bool PID, bool FULL>
 __global__ void __launch_bounds__(256) k_first_step(DPot P, IArgs A) {
   const long long stride = (long long)gridDim.x * blockDim.x;
@@ -200,8 +203,8 @@ __global__ void __launch_bounds__(256) k_first_step(DPot P, IArgs A) {
       const double* w = A.w0 + i * 6;
       const double q[3] = {w[0], w[1], w[2]};
       const double p[3] = {dir * w[3], dir * w[4], dir * w[5]};
-      h0 = initial_step<FULL>(P, dir, dir * t0v, q, p, A.rtol, A.atol,
-                              (double)ORDER);
+      h0 = initial_step<EvalTable<FULL>>(P, dir, dir * t0v, q, p, A.rtol,
+                                         A.atol, (double)ORDER);
     }
     if (PID) {
       h0 = fmin(h0, A.dtmax);
@@ -304,7 +307,20 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
         for (; si < A.T; ++si) write_save(A, orb, si, dir, q, p);
         done = true;
       } else {
-        const double h0 = A.h0[orb];
+        // the first step: dt0, or diffrax's automatic choice, clipped
+        double h0;
+        if constexpr (E::kInlineFirstStep) {
+          h0 = A.dt0;
+          if (!(h0 > 0.0))
+            h0 = initial_step<E>(P, dir, tau, q, p, A.rtol, A.atol,
+                                 (double)M::order);
+          if (PID) {
+            h0 = fmin(h0, A.dtmax);
+            if (A.dtmin > 0.0) h0 = fmax(h0, A.dtmin);
+          }
+        } else {
+          h0 = A.h0[orb];
+        }
         C.at_dtmin[L] = (PID && A.dtmin > 0.0 && h0 == A.dtmin) ? 1 : 0;
         tnext = fmin(tau + h0, dir * t1v);
         // saves at the start time
@@ -480,7 +496,6 @@ __global__ void k_reset_queue(unsigned long long* c, unsigned int* loc) {
 template <class M, bool PID, class E>
 int launch(const gx_pot* pot, IArgs& A, hipStream_t stream) {
   const auto kern = k_integrate<M, PID, E>;
-  constexpr bool FULL = E::kFrames;  // the prepass's generic field must see frames
   const LaunchShape& ls = launch_shape();
   const int block = kBlock;
   int occ = 0;
@@ -490,7 +505,9 @@ int launch(const gx_pot* pot, IArgs& A, hipStream_t stream) {
     occ = 1;
   int per_cu = ls.blocks_per_cu;
   if (per_cu > occ) per_cu = occ;
-  long long blocks = (long long)num_cus(pot->device) * per_cu;
+  // one launch takes 1/parts of the resident capacity, so that `parts`
+  // launches on different streams can run side by side
+  long long blocks = (long long)num_cus(pot->device) * per_cu / ls.parts;
   const long long need = (A.n + block - 1) / block;
   if (blocks > need) blocks = need;
   if (blocks < 1) blocks = 1;
@@ -503,13 +520,17 @@ int launch(const gx_pot* pot, IArgs& A, hipStream_t stream) {
   // most occ - 1 neighbours per head workgroup retire, and the condition
   // leaves at least head_blocks workgroups taking tickets.
   A.isolate = blocks >= (long long)A.head_blocks * (occ + 1) ? 1 : 0;
-  long long fblocks = (A.n + 255) / 256;
-  const long long fcap = (long long)num_cus(pot->device) * 8;
-  if (fblocks > fcap) fblocks = fcap;
   hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, stream, A.counter,
                      A.head_loc);
-  hipLaunchKernelGGL((k_first_step<M::order, PID, FULL>), dim3((unsigned)fblocks),
-                     dim3(256), 0, stream, pot->dev, A);
+  if constexpr (!E::kInlineFirstStep) {
+    constexpr bool FULL = E::kFrames;  // the prepass's field must see frames
+    long long fblocks = (A.n + 255) / 256;
+    const long long fcap = (long long)num_cus(pot->device) * 8;
+    if (fblocks > fcap) fblocks = fcap;
+    hipLaunchKernelGGL((k_first_step<M::order, PID, FULL>),
+                       dim3((unsigned)fblocks), dim3(256), 0, stream, pot->dev,
+                       A);
+  }
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), 0, stream,
                      pot->dev, A);
   return check(hipGetLastError());

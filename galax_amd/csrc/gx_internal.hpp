@@ -34,10 +34,12 @@ namespace gx {
 
 inline int check(hipError_t e) { return e == hipSuccess ? GX_OK : GX_ELAUNCH; }
 
-// Launch-shape defaults, overridable through gx_set_launch_shape.
+// Launch-shape defaults, overridable through gx_set_launch_shape and
+// gx_set_launch_fraction.
 struct LaunchShape {
   int block = 256;
   int blocks_per_cu = 4;
+  int parts = 1;  // a persistent launch takes 1/parts of the device
 };
 LaunchShape& launch_shape();
 

@@ -559,10 +559,17 @@ int gx_set_launch_shape(int32_t block, int32_t blocks_per_cu,
   LaunchShape& s = launch_shape();
   if (prev_block) *prev_block = s.block;
   if (prev_blocks_per_cu) *prev_blocks_per_cu = s.blocks_per_cu;
-  if (block < 0 || block > 1024 || (block % 64) != 0) return GX_EINVAL;
+  if (block != 0 && block != s.block) return GX_EINVAL;  // fixed at 256
   if (blocks_per_cu < 0 || blocks_per_cu > 64) return GX_EINVAL;
-  if (block > 0) s.block = block;
   if (blocks_per_cu > 0) s.blocks_per_cu = blocks_per_cu;
+  return GX_OK;
+}
+
+int gx_set_launch_fraction(int32_t parts, int32_t* prev_parts) {
+  LaunchShape& s = launch_shape();
+  if (prev_parts) *prev_parts = s.parts;
+  if (parts < 0 || parts > 64) return GX_EINVAL;
+  if (parts > 0) s.parts = parts;
   return GX_OK;
 }
 

@@ -605,6 +605,7 @@ template <int... K>
 struct EvalFixed {
   static constexpr bool kLarge = false;
   static constexpr bool kFrames = false;  // origin-centred table only
+  static constexpr bool kInlineFirstStep = false;
   template <size_t... I>
   __device__ __forceinline__ static void run(const DPot& P, double x, double y,
                                              double z, double& ax, double& ay,
@@ -629,6 +630,7 @@ template <bool FULL>
 struct EvalTable {
   static constexpr bool kLarge = FULL;
   static constexpr bool kFrames = FULL;
+  static constexpr bool kInlineFirstStep = false;
   __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
                                              double z, double t, double& ax,
                                              double& ay, double& az) {
@@ -646,6 +648,8 @@ struct EvalTable {
 struct EvalMW {
   static constexpr bool kLarge = false;
   static constexpr bool kFrames = false;
+  // the first step is computed in the integrator (small evaluator)
+  static constexpr bool kInlineFirstStep = true;
   __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
                                              double z, double /*t*/,
                                              double& ax, double& ay,
@@ -688,6 +692,7 @@ using EvalMilkyWay = EvalMW;
 struct EvalMilkyWayRotating {
   static constexpr bool kLarge = true;  // rolled stage loop: one copy of the group loop
   static constexpr bool kFrames = true;
+  static constexpr bool kInlineFirstStep = false;
   __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
                                              double z, double t, double& ax,
                                              double& ay, double& az) {
@@ -721,6 +726,7 @@ struct EvalMilkyWayRotating {
 struct EvalMilkyWayTracked {
   static constexpr bool kLarge = false;
   static constexpr bool kFrames = true;
+  static constexpr bool kInlineFirstStep = false;
   __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
                                              double z, double t, double& ax,
                                              double& ay, double& az) {

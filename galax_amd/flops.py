@@ -46,3 +46,10 @@ def f_save(method: str = "dopri8", f_rhs: int = F_RHS_MW) -> int:
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
+
+# HBM bytes of one k_integrate pass, measured with rocprofv3 (FETCH_SIZE and
+# WRITE_SIZE, each scaled by the same counter's reading for a plain 1 GiB
+# copy: x2.0 and x1.0), for bench.py's workloads at 1,048,576 orbits and 64
+# saves.  The throughput workload's measurement and commands are kept in
+# profiles/k_integrate_mw_uneven_traffic.txt.
+MEASURED_TRAFFIC_BYTES = {"uneven": 5.4314e9}

@@ -281,12 +281,24 @@ int gx_adjoint(const gx_pot* pot, const gx_solver_cfg* cfg, const double* w0,
                int32_t ntheta, int64_t max_steps, int32_t K, int32_t* status,
                void* workspace, void* stream);
 
-/* Launch-shape override used by the autotuner: workgroup size (multiple of
- * 64, <= 1024) and workgroups per CU for the persistent integrator; 0 keeps
- * the built-in default.  Returns the previous values through the pointers
- * when they are non-NULL.  Not thread-safe with concurrent launches. */
+/* Launch shape of the persistent integrator: resident workgroups per CU
+ * that a launch may take (capped by the kernel's occupancy; 0 keeps the
+ * current value).  `block` must be 0 or 256 (the kernel's fixed workgroup
+ * size).  Returns the previous values through the pointers when they are
+ * non-NULL.  A process-wide setting: not thread-safe with concurrent
+ * launches.  No reference counterpart (a tuning control). */
 int gx_set_launch_shape(int32_t block, int32_t blocks_per_cu,
                         int32_t* prev_block, int32_t* prev_blocks_per_cu);
+
+/* Each persistent integrator launch takes 1/parts of the device's resident
+ * capacity (parts >= 1; 0 keeps the current value; default 1).  A launch is
+ * bounded by its longest orbit's serial step chain, during which most of
+ * the device falls idle; a caller that keeps `parts` independent batches in
+ * flight on as many streams runs them side by side.  The hardware queues of
+ * a process bound the useful value (4 by default).  Returns the previous
+ * value through `prev_parts` when non-NULL.  Process-wide, like
+ * gx_set_launch_shape.  No reference counterpart (a tuning control). */
+int gx_set_launch_fraction(int32_t parts, int32_t* prev_parts);
 
 /* Library and device information for tooling: number of compute units of
  * the current device, and the VGPR count of the integrator kernels. */
