@@ -228,22 +228,40 @@ struct Cold {
   int si[kBlock], st[kBlock], at_dtmin[kBlock];
 };
 
-// True when a head workgroup has recorded `loc` (agent-scope loads: the
-// records are written by other workgroups during this launch).
-__device__ __forceinline__ bool on_head_cu(const IArgs& A, unsigned int loc) {
-  for (int k = 0; k < A.head_blocks; ++k)
-    if (__hip_atomic_load(&A.head_loc[k], __ATOMIC_RELAXED,
-                          __HIP_MEMORY_SCOPE_AGENT) == loc)
-      return true;
-  return false;
+// Whether a lane outside the head keeps taking tickets.  Head workgroups
+// record their compute unit and then count themselves in (head_loc[k],
+// k < head_blocks, and the counter head_loc[kMaxHeadBlocks]).  A lane reads
+// all the records at once (independent agent-scope loads, one round trip)
+// and settles the question for good once every head workgroup has counted
+// itself in: iso = 1 keep working, 2 retire; 0 while undecided.
+__device__ __forceinline__ void settle_isolation(const IArgs& A,
+                                                 unsigned int loc, int& iso) {
+  const unsigned int ready =
+      __hip_atomic_load(&A.head_loc[kMaxHeadBlocks], __ATOMIC_ACQUIRE,
+                        __HIP_MEMORY_SCOPE_AGENT);
+  bool hit = false;
+#pragma unroll
+  for (int k = 0; k < kMaxHeadBlocks; ++k)
+    if (k < A.head_blocks)
+      hit = hit || (__hip_atomic_load(&A.head_loc[k], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT) == loc);
+  if (hit)
+    iso = 2;
+  else if (ready >= (unsigned int)A.head_blocks)
+    iso = 1;
 }
 
 // The lane's next ticket, or -1: the queue is empty, or (when isolating
 // the head) the lane is a head lane, or shares a head workgroup's compute
 // unit.
 __device__ __forceinline__ long long next_orbit(const IArgs& A,
-                                                unsigned int loc, bool head) {
-  if (A.isolate && (head || on_head_cu(A, loc))) return -1;
+                                                unsigned int loc, bool head,
+                                                int& iso) {
+  if (A.isolate) {
+    if (head) return -1;
+    if (iso == 0) settle_isolation(A, loc, iso);
+    if (iso == 2) return -1;
+  }
   const long long idx =
       A.head_lanes + (long long)atomicAdd(A.counter, 1ULL);
   return ticket_to_orbit(A, idx);
@@ -258,17 +276,21 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
   const bool head = blockIdx.x < (unsigned)A.head_blocks;
   if (threadIdx.x == 0) {
     s_loc = cu_location();
-    if (head)
+    if (head) {
       __hip_atomic_store(&A.head_loc[blockIdx.x], s_loc, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&A.head_loc[kMaxHeadBlocks], 1u, __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
   __syncthreads();
   const unsigned int loc = s_loc;
+  int iso = 0;
   // The first tickets of a cost-ordered queue hold the longest orbits, whose
   // step latency bounds the launch: their waves start on them, issue ahead
   // of any other wave on their SIMD, and keep the SIMD to themselves once
   // the workgroups beside them stop taking tickets.
-  long long orb = head ? ticket_to_orbit(A, lane) : next_orbit(A, loc, false);
+  long long orb = head ? ticket_to_orbit(A, lane) : next_orbit(A, loc, false, iso);
   bool fresh = true;
   if (head) __builtin_amdgcn_s_setprio(3);
 
@@ -479,18 +501,19 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
       if (A.status) A.status[orb] = st;
       if (A.nsteps) A.nsteps[orb] = C.nst[L];
       if (A.naccept) A.naccept[orb] = C.nacc[L];
-      orb = next_orbit(A, loc, head);
+      orb = next_orbit(A, loc, head, iso);
       fresh = true;
     }
   }
 }
 
-// Clear the queue counter and the head-location records (a kernel, not a
-// memset, so that a captured launch sequence replays correctly).
+// Clear the queue counter, the head-location records and their counter (a
+// kernel, not a memset, so that a captured launch sequence replays
+// correctly).
 __global__ void k_reset_queue(unsigned long long* c, unsigned int* loc) {
   if (blockIdx.x != 0) return;
   if (threadIdx.x == 0) *c = 0ULL;
-  if (threadIdx.x < kMaxHeadBlocks) loc[threadIdx.x] = 0u;
+  if (threadIdx.x <= kMaxHeadBlocks) loc[threadIdx.x] = 0u;
 }
 
 template <class M, bool PID, class E>

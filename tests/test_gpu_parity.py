@@ -284,6 +284,45 @@ def test_queue_head_isolation_does_not_change_results():
     check_adaptive(oa[idx], ref, a.nsteps.cpu().numpy()[idx], ns, all_tol=1e-6)
 
 
+def test_batches_in_flight_do_not_change_results():
+    """Launches that take a quarter of the device each, several in flight on
+    separate streams at once, give bitwise the results of one whole-device
+    launch."""
+    import torch
+
+    from galax_amd import _lib
+
+    n = 20480
+    w0, t0, T = workloads.config_uneven(n=n, seed=13)
+    T = np.minimum(T, 120.0)
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.PIDController(rtol=1e-8, atol=1e-10), max_steps=None)
+    pot = gp.MilkyWayPotential()
+    lib = _lib.lib()
+    say("batches in flight", n)
+    ref = gd.integrate_orbits(pot, w0, t0, T, T=3, solver=solver, order=workloads.cost_order(w0, T))
+    ref_out = ref.out.cpu().numpy()
+    prev = __import__("ctypes").c_int32(0)
+    _lib.check(lib.gx_set_launch_fraction(4, __import__("ctypes").byref(prev)), "gx_set_launch_fraction")
+    try:
+        streams = [torch.cuda.Stream() for _ in range(3)]
+        main = torch.cuda.current_stream()
+        w0d, t0d, Td = (torch.as_tensor(a, device="cuda") for a in (w0, t0, T))
+        od = torch.as_tensor(workloads.cost_order(w0, T), device="cuda")
+        results = []
+        for s_ in streams:
+            s_.wait_stream(main)
+            with torch.cuda.stream(s_):
+                results.append(gd.integrate_orbits(pot, w0d, t0d, Td, T=3, solver=solver, order=od))
+        for s_ in streams:
+            main.wait_stream(s_)
+        torch.cuda.synchronize()
+    finally:
+        _lib.check(lib.gx_set_launch_fraction(prev.value, None), "gx_set_launch_fraction")
+    for r in results:
+        np.testing.assert_array_equal(r.out.cpu().numpy(), ref_out)
+        np.testing.assert_array_equal(r.nsteps.cpu().numpy(), ref.nsteps.cpu().numpy())
+
+
 def test_evaluate_orbit_two_stage():
     pot = gp.KeplerPotential(m_tot=1e12)
     say("evaluate_orbit two stage")
