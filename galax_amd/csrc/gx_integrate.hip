@@ -14,10 +14,8 @@
 //     ticket (a device-scope atomic, then dependent loads of the queue
 //     entry and the new orbit's state), so in a large grid the first
 //     kHeadLanes lanes integrate tickets 0..kHeadLanes-1 and take no more:
-//     their waves run the bare step loop at top wave priority.  Workgroups
-//     that find themselves on a head workgroup's compute unit (read from
-//     the hardware registers) take no tickets either, so the head waves own
-//     their SIMDs.  The rest of the grid drains the queue;
+//     their waves run the bare step loop at top wave priority, and the rest
+//     of the grid drains the queue;
 //   * the state, the stage accelerations (Runge-Kutta-Nystrom form, 3
 //     doubles per stage), the error norm and the PID controller live in
 //     registers; all arithmetic is FP64;
@@ -59,25 +57,13 @@ struct IArgs {
   long long* naccept;
   unsigned long long* counter;
   double* h0;  // first step per orbit, from the prepass (workspace)
-  unsigned int* head_loc;  // compute unit of each head workgroup (workspace)
-  long long head_lanes;    // lanes that start on a fixed ticket
-  int head_blocks;
-  int isolate;  // head lanes take one ticket; their neighbours none
+  long long head_lanes;  // lanes that start on a fixed ticket
+  int head_only_once;    // head lanes take no second ticket
 };
 
 constexpr int kBlock = 256;
 constexpr long long kStepCap = 10000000;  // attempted steps per orbit
 constexpr long long kHeadLanes = 4096;     // lanes that start on the queue head
-constexpr int kMaxHeadBlocks = 32;         // head-location slots in the workspace
-
-// Where this wave runs: its XCD and its compute unit within it (the
-// HW_ID fields CU_ID, SH_ID, SE_ID), never 0.  Read-only hardware registers.
-__device__ __forceinline__ unsigned int cu_location() {
-  const unsigned int hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);    // HW_ID
-  const unsigned int xcc = __builtin_amdgcn_s_getreg((15 << 11) | 20);  // XCC_ID
-  return 0x80000000u | ((xcc & 0xFu) << 16) | (hw & 0xFF00u);
-}
-
 // diffrax's automatic first step (Hairer, Norsett & Wanner II.4; final
 // exponent 1/(p+1) for a method of order p) for one orbit in solver
 // variables, with the integrator's own field evaluator.  Two field
@@ -228,40 +214,10 @@ struct Cold {
   int si[kBlock], st[kBlock], at_dtmin[kBlock];
 };
 
-// Whether a lane outside the head keeps taking tickets.  Head workgroups
-// record their compute unit and then count themselves in (head_loc[k],
-// k < head_blocks, and the counter head_loc[kMaxHeadBlocks]).  A lane reads
-// all the records at once (independent agent-scope loads, one round trip)
-// and settles the question for good once every head workgroup has counted
-// itself in: iso = 1 keep working, 2 retire; 0 while undecided.
-__device__ __forceinline__ void settle_isolation(const IArgs& A,
-                                                 unsigned int loc, int& iso) {
-  const unsigned int ready =
-      __hip_atomic_load(&A.head_loc[kMaxHeadBlocks], __ATOMIC_ACQUIRE,
-                        __HIP_MEMORY_SCOPE_AGENT);
-  bool hit = false;
-#pragma unroll
-  for (int k = 0; k < kMaxHeadBlocks; ++k)
-    if (k < A.head_blocks)
-      hit = hit || (__hip_atomic_load(&A.head_loc[k], __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT) == loc);
-  if (hit)
-    iso = 2;
-  else if (ready >= (unsigned int)A.head_blocks)
-    iso = 1;
-}
-
-// The lane's next ticket, or -1: the queue is empty, or (when isolating
-// the head) the lane is a head lane, or shares a head workgroup's compute
-// unit.
-__device__ __forceinline__ long long next_orbit(const IArgs& A,
-                                                unsigned int loc, bool head,
-                                                int& iso) {
-  if (A.isolate) {
-    if (head) return -1;
-    if (iso == 0) settle_isolation(A, loc, iso);
-    if (iso == 2) return -1;
-  }
+// The lane's next ticket, or -1: the queue is empty, or the lane is a head
+// lane of a grid large enough for the rest to drain the queue.
+__device__ __forceinline__ long long next_orbit(const IArgs& A, bool head) {
+  if (head && A.head_only_once) return -1;
   const long long idx =
       A.head_lanes + (long long)atomicAdd(A.counter, 1ULL);
   return ticket_to_orbit(A, idx);
@@ -270,27 +226,13 @@ __device__ __forceinline__ long long next_orbit(const IArgs& A,
 template <class M, bool PID, class E>
 __global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
   __shared__ Cold C;
-  __shared__ unsigned int s_loc;
   const int L = threadIdx.x;
   const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool head = blockIdx.x < (unsigned)A.head_blocks;
-  if (threadIdx.x == 0) {
-    s_loc = cu_location();
-    if (head) {
-      __hip_atomic_store(&A.head_loc[blockIdx.x], s_loc, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&A.head_loc[kMaxHeadBlocks], 1u, __ATOMIC_RELEASE,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-  __syncthreads();
-  const unsigned int loc = s_loc;
-  int iso = 0;
   // The first tickets of a cost-ordered queue hold the longest orbits, whose
-  // step latency bounds the launch: their waves start on them, issue ahead
-  // of any other wave on their SIMD, and keep the SIMD to themselves once
-  // the workgroups beside them stop taking tickets.
-  long long orb = head ? ticket_to_orbit(A, lane) : next_orbit(A, loc, false, iso);
+  // step latency bounds the launch: their lanes start on them, and their
+  // waves issue ahead of any other wave on their SIMD.
+  const bool head = lane < A.head_lanes;
+  long long orb = head ? ticket_to_orbit(A, lane) : next_orbit(A, false);
   bool fresh = true;
   if (head) __builtin_amdgcn_s_setprio(3);
 
@@ -501,19 +443,16 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
       if (A.status) A.status[orb] = st;
       if (A.nsteps) A.nsteps[orb] = C.nst[L];
       if (A.naccept) A.naccept[orb] = C.nacc[L];
-      orb = next_orbit(A, loc, head, iso);
+      orb = next_orbit(A, head);
       fresh = true;
     }
   }
 }
 
-// Clear the queue counter, the head-location records and their counter (a
-// kernel, not a memset, so that a captured launch sequence replays
-// correctly).
-__global__ void k_reset_queue(unsigned long long* c, unsigned int* loc) {
-  if (blockIdx.x != 0) return;
-  if (threadIdx.x == 0) *c = 0ULL;
-  if (threadIdx.x <= kMaxHeadBlocks) loc[threadIdx.x] = 0u;
+// Clear the queue counter (a kernel, not a memset, so that a captured
+// launch sequence replays correctly).
+__global__ void k_reset_queue(unsigned long long* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ULL;
 }
 
 template <class M, bool PID, class E>
@@ -534,17 +473,11 @@ int launch(const gx_pot* pot, IArgs& A, hipStream_t stream) {
   const long long need = (A.n + block - 1) / block;
   if (blocks > need) blocks = need;
   if (blocks < 1) blocks = 1;
-  A.head_blocks = (int)(kHeadLanes / block);
-  if (A.head_blocks > kMaxHeadBlocks) A.head_blocks = kMaxHeadBlocks;
-  if (A.head_blocks > blocks) A.head_blocks = (int)blocks;
-  A.head_lanes = (long long)A.head_blocks * block;
-  // Isolation needs a grid whose other workgroups certainly drain the
-  // queue: a compute unit holds at most `occ` of these workgroups, so at
-  // most occ - 1 neighbours per head workgroup retire, and the condition
-  // leaves at least head_blocks workgroups taking tickets.
-  A.isolate = blocks >= (long long)A.head_blocks * (occ + 1) ? 1 : 0;
-  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, stream, A.counter,
-                     A.head_loc);
+  // Head lanes take one ticket only when the rest of the grid is large
+  // enough to drain the queue (at least three times the head).
+  A.head_lanes = kHeadLanes < blocks * block ? kHeadLanes : blocks * block;
+  A.head_only_once = blocks * block >= 4 * kHeadLanes ? 1 : 0;
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, stream, A.counter);
   if constexpr (!E::kInlineFirstStep) {
     constexpr bool FULL = E::kFrames;  // the prepass's field must see frames
     long long fblocks = (A.n + 255) / 256;
@@ -626,7 +559,6 @@ int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
   A.nsteps = reinterpret_cast<long long*>(nsteps);
   A.naccept = reinterpret_cast<long long*>(naccept);
   A.counter = static_cast<unsigned long long*>(workspace);
-  A.head_loc = reinterpret_cast<unsigned int*>(static_cast<char*>(workspace) + 64);
   A.h0 = reinterpret_cast<double*>(static_cast<char*>(workspace) + 256);
 
   hipStream_t s = (hipStream_t)stream;

@@ -224,8 +224,7 @@ enum gx_ts_mode {
  * shortens the tail of a launch of uneven orbits.
  * `workspace` must point to at least gx_integrate_workspace_bytes(n) bytes of
  * device memory (256-byte aligned) that no concurrent call uses; it holds the
- * work-queue counter, the compute units of the queue-head workgroups and
- * each orbit's first step. */
+ * work-queue counter and each orbit's first step. */
 int64_t gx_integrate_workspace_bytes(int64_t n);
 int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
                  const double* w0, const double* t0, const double* t1,

@@ -262,16 +262,16 @@ def test_queue_order_does_not_change_results():
     np.testing.assert_array_equal(a.out.cpu().numpy(), b.out.cpu().numpy())
 
 
-def test_queue_head_isolation_does_not_change_results():
+def test_queue_head_policy_does_not_change_results():
     """A grid large enough for the queue-head policy (head lanes take one
-    ticket, their compute-unit neighbours none, the rest drain the queue):
+    ticket, the rest drain the queue):
     every orbit is integrated, bitwise as with another queue order, and as
     the oracle integrates it."""
     n = 24576
     w0, t0, T = workloads.config_uneven(n=n, seed=12)
     T = np.minimum(T, 150.0)
     solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.PIDController(rtol=1e-8, atol=1e-10), max_steps=None)
-    say("queue head isolation", n)
+    say("queue head policy", n)
     a = gd.integrate_orbits(gp.MilkyWayPotential(), w0, t0, T, T=3, solver=solver, order=workloads.cost_order(w0, T))
     b = gd.integrate_orbits(gp.MilkyWayPotential(), w0, t0, T, T=3, solver=solver,
                             order=np.random.default_rng(3).permutation(n))
