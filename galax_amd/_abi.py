@@ -31,6 +31,7 @@ GX_LEESUTO_NFW = 19
 GX_VOGELSBERGER_NFW = 20
 
 GX_MAX_GL_ORDER = 128
+GX_MATH_CHECK_OUTPUTS = 9
 
 GX_FRAME_NONE = 0
 GX_FRAME_OFFSET = 1

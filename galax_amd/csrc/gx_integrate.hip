@@ -67,16 +67,19 @@ constexpr long long kHeadLanes = 4096;     // lanes that start on the queue head
 // diffrax's automatic first step (Hairer, Norsett & Wanner II.4; final
 // exponent 1/(p+1) for a method of order p) for one orbit in solver
 // variables, with the integrator's own field evaluator.  Two field
-// evaluations, one call site.
-template <class E>
+// evaluations, one call site.  Quotients, square roots and the final root
+// use the short primitives (each within about an ulp of the correctly
+// rounded operation): a lane that starts an orbit runs this while the rest
+// of its wave waits.
+template <class E, int ORDER>
 __device__ double initial_step(const DPot& P, double dir, double tau0,
                                const double (&q)[3], const double (&p)[3],
-                               double rtol, double atol, double order) {
-  double sc[6];
+                               double rtol, double atol) {
+  double isc[6];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    sc[m] = atol + fabs(q[m]) * rtol;
-    sc[m + 3] = atol + fabs(p[m]) * rtol;
+    isc[m] = rcp(atol + fabs(q[m]) * rtol);
+    isc[m + 3] = rcp(atol + fabs(p[m]) * rtol);
   }
   double a0[3] = {0.0, 0.0, 0.0}, q1[3], p1[3];
   double d1 = 0.0, h0 = 0.0, d2 = 0.0;
@@ -98,15 +101,15 @@ __device__ double initial_step(const DPot& P, double dir, double tau0,
       double d0 = 0.0;
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
-        const double u = q[m] / sc[m], v = p[m] / sc[m + 3];
+        const double u = q[m] * isc[m], v = p[m] * isc[m + 3];
         d0 += u * u + v * v;
-        const double fu = p[m] / sc[m], fv = a0[m] / sc[m + 3];
+        const double fu = p[m] * isc[m], fv = a0[m] * isc[m + 3];
         d1 += fu * fu + fv * fv;
       }
-      d0 = sqrt(d0 / 6.0);
-      d1 = sqrt(d1 / 6.0);
+      d0 = sqrt_fast(div6(d0));
+      d1 = sqrt_fast(div6(d1));
       const bool cond = (d0 < 1e-5) || (d1 < 1e-5);
-      h0 = cond ? 1e-6 : 0.01 * (d0 / (cond ? 1.0 : d1));
+      h0 = cond ? 1e-6 : 0.01 * div(d0, cond ? 1.0 : d1);
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
         q1[m] = q[m] + p[m] * h0;
@@ -116,16 +119,16 @@ __device__ double initial_step(const DPot& P, double dir, double tau0,
       const double a1[3] = {ax, ay, az};
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
-        const double u = (p1[m] - p[m]) / sc[m],
-                     v = (a1[m] - a0[m]) / sc[m + 3];
+        const double u = (p1[m] - p[m]) * isc[m],
+                     v = (a1[m] - a0[m]) * isc[m + 3];
         d2 += u * u + v * v;
       }
-      d2 = sqrt(d2 / 6.0) / h0;
+      d2 = div(sqrt_fast(div6(d2)), h0);
     }
   }
   const double maxd = fmax(d1, d2);
   const double h1 = (maxd <= 1e-15) ? fmax(1e-6, h0 * 1e-3)
-                                    : pow(0.01 / maxd, 1.0 / (order + 1.0));
+                                    : root_n<ORDER + 1>(div(0.01, maxd));
   return fmin(100.0 * h0, h1);
 }
 
@@ -189,8 +192,8 @@ __global__ void __launch_bounds__(256) k_first_step(DPot P, IArgs A) {
       const double* w = A.w0 + i * 6;
       const double q[3] = {w[0], w[1], w[2]};
       const double p[3] = {dir * w[3], dir * w[4], dir * w[5]};
-      h0 = initial_step<EvalTable<FULL>>(P, dir, dir * t0v, q, p, A.rtol,
-                                         A.atol, (double)ORDER);
+      h0 = initial_step<EvalTable<FULL>, ORDER>(P, dir, dir * t0v, q, p,
+                                                A.rtol, A.atol);
     }
     if (PID) {
       h0 = fmin(h0, A.dtmax);
@@ -276,8 +279,8 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
         if constexpr (E::kInlineFirstStep) {
           h0 = A.dt0;
           if (!(h0 > 0.0))
-            h0 = initial_step<E>(P, dir, tau, q, p, A.rtol, A.atol,
-                                 (double)M::order);
+            h0 = initial_step<E, M::order>(P, dir, tau, q, p, A.rtol,
+                                           A.atol);
           if (PID) {
             h0 = fmin(h0, A.dtmax);
             if (A.dtmin > 0.0) h0 = fmax(h0, A.dtmin);
@@ -343,23 +346,35 @@ __global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
             bool keep = true;
             double next_t1;
             if (PID) {
+              // diffrax's RMS norm of the scaled error, here through its
+              // mean square ms (the norm is below 1 exactly when ms is);
+              // quotients and roots by the short primitives, within about
+              // an ulp of the correctly rounded operations
               double sq = 0.0;
 #pragma unroll
               for (int m = 0; m < 3; ++m) {
                 const double uq =
-                    eq[m] / (A.atol + fmax(fabs(q[m]), fabs(q1[m])) * A.rtol);
+                    eq[m] * rcp(A.atol + fmax(fabs(q[m]), fabs(q1[m])) * A.rtol);
                 const double up =
-                    ep[m] / (A.atol + fmax(fabs(p[m]), fabs(p1[m])) * A.rtol);
+                    ep[m] * rcp(A.atol + fmax(fabs(p[m]), fabs(p1[m])) * A.rtol);
                 sq += uq * uq + up * up;
               }
-              const double scaled = sqrt(sq / 6.0);
-              keep = scaled < 1.0;
+              const double ms = div6(sq);
+              keep = ms < 1.0;
               const bool at_dtmin = C.at_dtmin[L] != 0;
               if (A.dtmin > 0.0) keep = keep || at_dtmin;
-              const double inv = 1.0 / scaled;
+              // inv = 1 / norm = ms^(-1/2); zero (and subnormal) ms gives
+              // +inf, infinite ms 0, NaN stays NaN
+              const bool normal = ms >= kMinNormal && ms <= kMaxFinite;
+              const double special = (ms < kMinNormal)   ? __builtin_inf()
+                                     : (ms > kMaxFinite) ? 0.0
+                                                         : ms;
+              const double a = rsqrt_fast(ms);
+              const double inv = normal ? a : special;
               double f;
               if (A.root8) {
-                f = sqrt(sqrt(sqrt(inv)));
+                // inv^(1/8) = 1 / rsqrt(rsqrt(rsqrt(inv)))
+                f = normal ? rcp(rsqrt_fast(rsqrt_fast(rsqrt_fast(a)))) : special;
               } else {
                 f = (A.c1 == 0.0) ? 1.0 : pow(inv, A.c1);
                 if (A.c2 != 0.0) f *= pow(C.prev_inv[L], A.c2);

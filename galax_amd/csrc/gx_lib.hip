@@ -300,16 +300,23 @@ __global__ __launch_bounds__(kFieldBlock) void k_potential(
 }
 
 // The short FP64 primitives of gx_math.hpp at given arguments (accuracy
-// checks): out[4i..4i+3] = rcp(x), rsqrt_fast(x), log1p_fast(x), div(1, x).
+// checks), GX_MATH_CHECK_OUTPUTS values per argument in the order the
+// header lists them.
 __global__ void k_math_check(const double* __restrict__ x, long long n,
                              double* __restrict__ out) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double v = x[i];
-  out[4 * i] = rcp(v);
-  out[4 * i + 1] = rsqrt_fast(v);
-  out[4 * i + 2] = log1p_fast(v);
-  out[4 * i + 3] = div(1.0, v);
+  double* o = out + GX_MATH_CHECK_OUTPUTS * i;
+  o[0] = rcp(v);
+  o[1] = rsqrt_fast(v);
+  o[2] = log1p_fast(v);
+  o[3] = div(1.0, v);
+  o[4] = sqrt_fast(v);
+  o[5] = div6(v);
+  o[6] = root_n<9>(v);
+  o[7] = root_n<6>(v);
+  o[8] = rcp(rsqrt_fast(rsqrt_fast(rsqrt_fast(v))));
 }
 
 __global__ __launch_bounds__(kFieldBlock) void k_hessian(

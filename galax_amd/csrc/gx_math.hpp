@@ -46,6 +46,67 @@ __device__ __forceinline__ double div(double n, double d) {
   return fma(fma(-d, q, n), r, q);
 }
 
+constexpr double kMinNormal = 2.2250738585072014e-308;
+constexpr double kMaxFinite = 1.7976931348623157e308;
+
+// sqrt(x) for x >= 0 or NaN: s = x / sqrt(x) from the estimate above, then
+// one residual correction s + (x - s^2) / (2 sqrt(x)) (within 0.6 ulp) for
+// positive normal finite x; 0 -> 0, +inf -> +inf, NaN -> NaN, and a
+// subnormal x gives 0 (below 1.5e-154, which the callers treat as 0).
+__device__ __forceinline__ double sqrt_fast(double x) {
+  const double r = rsqrt_fast(x);
+  const double s = x * r;
+  const double c = fma(0.5 * r, fma(-s, s, x), s);
+  return (x >= kMinNormal && x <= kMaxFinite) ? c : (x > kMaxFinite ? x : 0.0 * x);
+}
+
+// x / 6 by the product with the rounded 1/6 and one residual correction
+// (the correctly rounded quotient except in rare double-rounding cases);
+// non-finite x as the division gives it.
+__device__ __forceinline__ double div6(double x) {
+  constexpr double kSixth = 1.0 / 6.0;
+  const double q = x * kSixth;
+  const double c = fma(fma(-6.0, q, x), kSixth, q);
+  return (q >= -kMaxFinite && q <= kMaxFinite) ? c : q;
+}
+
+// y^N by repeated squaring.
+template <int N>
+__device__ __forceinline__ double ipow(double y) {
+  if constexpr (N == 1) {
+    return y;
+  } else if constexpr (N % 2 == 0) {
+    const double h = ipow<N / 2>(y);
+    return h * h;
+  } else {
+    return ipow<N - 1>(y) * y;
+  }
+}
+
+// x^(1/N) for x >= 0 or NaN (within 1.25 ulp of the exact root): with x = m 2^k,
+// m in [0.5, 1) and k = N j + r, 0 <= r < N, x^(1/N) = 2^j (m 2^r)^(1/N).
+// The root of w = m 2^r in [0.5, 2^(N-1)) starts from the single-precision
+// hardware log2/exp2 (relative error near 1e-7); two Newton steps on
+// y^N = w, y <- y + (y/N)(w / y^N - 1), each square the relative error.
+template <int N>
+__device__ __forceinline__ double root_n(double x) {
+  int k;
+  const double m = frexp(x, &k);
+  int j = k / N;
+  int r = k - N * j;
+  if (r < 0) {
+    r += N;
+    j -= 1;
+  }
+  const double w = ldexp(m, r);
+  double y = (double)__builtin_amdgcn_exp2f(
+      __builtin_amdgcn_logf((float)w) * (1.0f / (float)N));
+#pragma unroll
+  for (int it = 0; it < 2; ++it)
+    y = fma(y * (1.0 / N), fma(w, rcp(ipow<N>(y)), -1.0), y);
+  return (x > 0.0 && x <= kMaxFinite) ? ldexp(y, j) : x;
+}
+
 // log1p(x) given inv_u ~ 1/(1 + x) to within a few ulp (the caller often
 // has it already); used only for the rounding correction of 1 + x.
 __device__ __forceinline__ double log1p_inv(double x, double inv_u);
@@ -68,7 +129,9 @@ __device__ __forceinline__ double log1p_inv(double x, double inv_u) {
     k -= 1;
   }
   const double f = m - 1.0;
-  const double s = div(f, 2.0 + f);
+  // s enters the result only through s (f^2/2 + R), near f^3/4 against a
+  // result near f, so its 1.5-ulp quotient costs under 0.1 ulp
+  const double s = f * rcp(2.0 + f);
   const double z = s * s;
   // R(z) = sum_{n=1..10} 2 z^n / (2n + 1) by Estrin's scheme: the same terms
   // as Horner's rule with a dependency depth of 5 instead of 10, which is

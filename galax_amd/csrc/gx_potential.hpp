@@ -672,10 +672,12 @@ struct EvalMW {
     const double dh = r + hk[1], db = r + bk[1], dn = r + nk[1];
     const double dbn = db * dn;
     const double R = rcp(dh * dbn);
-    const double uh = dbn * R, ub = dh * dn * R, un = dh * db * R;
+    const double dR = dh * R;
+    const double uh = dbn * R, ub = dn * dR, un = db * dR;
     const double L = log1p_inv(r * hk[2], hk[1] * uh);
-    const double g = hk[0] * ir * ir * (L * ir - uh) + bk[0] * ub * ub * ir +
-                     nk[0] * un * un * ir;
+    const double g =
+        ir * fma(nk[0] * un, un,
+                 fma(bk[0] * ub, ub, (hk[0] * ir) * fma(L, ir, -uh)));
     ax = -(gd + g) * x;
     ay = -(gd + g) * y;
     az = -(gd * A * iS + g) * z;

@@ -23,12 +23,16 @@ __device__ __forceinline__ void rk_eval(const DPot& P, double dir, double tb,
                                         double (&ep)[3]) {
   constexpr int S = M::S;
   double g[S][3];
+  // Stage positions as q + rs (h p) + (h h) s: two FMAs per component
+  const double hh = h * h;
+  const double hp[3] = {h * p[0], h * p[1], h * p[2]};
   if constexpr (!E::kLarge) {
     // Small evaluator: unroll the stages; every index below is static.
     E::acc(P, q[0], q[1], q[2], dir * tb, g[0][0], g[0][1], g[0][2]);
 #pragma unroll
     for (int i = 1; i < S; ++i) {
       double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+      bool any = false;
 #pragma unroll
       for (int l = 0; l < i; ++l) {
         if (M::k.aa[i][l] != 0.0) {
@@ -36,13 +40,19 @@ __device__ __forceinline__ void rk_eval(const DPot& P, double dir, double tb,
           s0 = fma(a, g[l][0], s0);
           s1 = fma(a, g[l][1], s1);
           s2 = fma(a, g[l][2], s2);
+          any = true;
         }
       }
       const double rs = M::k.rs[i];
-      E::acc(P, fma(h, fma(rs, p[0], h * s0), q[0]),
-             fma(h, fma(rs, p[1], h * s1), q[1]),
-             fma(h, fma(rs, p[2], h * s2), q[2]), dir * fma(M::k.c[i], h, tb),
-             g[i][0], g[i][1], g[i][2]);
+      double Q0 = fma(rs, hp[0], q[0]), Q1 = fma(rs, hp[1], q[1]),
+             Q2 = fma(rs, hp[2], q[2]);
+      if (any) {
+        Q0 = fma(hh, s0, Q0);
+        Q1 = fma(hh, s1, Q1);
+        Q2 = fma(hh, s2, Q2);
+      }
+      E::acc(P, Q0, Q1, Q2, dir * fma(M::k.c[i], h, tb), g[i][0], g[i][1],
+             g[i][2]);
     }
   } else {
     // Large evaluator: keep the stage loop rolled so that it is emitted
@@ -72,9 +82,9 @@ __device__ __forceinline__ void rk_eval(const DPot& P, double dir, double tb,
             c = M::k.c[k];
           }
         }
-        Qx = fma(h, fma(rs, p[0], h * s0), q[0]);
-        Qy = fma(h, fma(rs, p[1], h * s1), q[1]);
-        Qz = fma(h, fma(rs, p[2], h * s2), q[2]);
+        Qx = fma(hh, s0, fma(rs, hp[0], q[0]));
+        Qy = fma(hh, s1, fma(rs, hp[1], q[1]));
+        Qz = fma(hh, s2, fma(rs, hp[2], q[2]));
         ts = fma(c, h, tb);
       }
       double ax, ay, az;
@@ -125,16 +135,16 @@ __device__ __forceinline__ void rk_eval(const DPot& P, double dir, double tb,
   p1[0] = fma(h, b0, p[0]);
   p1[1] = fma(h, b1, p[1]);
   p1[2] = fma(h, b2, p[2]);
-  q1[0] = fma(h, fma(sb, p[0], h * ba0), q[0]);
-  q1[1] = fma(h, fma(sb, p[1], h * ba1), q[1]);
-  q1[2] = fma(h, fma(sb, p[2], h * ba2), q[2]);
+  q1[0] = fma(hh, ba0, fma(sb, hp[0], q[0]));
+  q1[1] = fma(hh, ba1, fma(sb, hp[1], q[1]));
+  q1[2] = fma(hh, ba2, fma(sb, hp[2], q[2]));
   if (ERR) {
     ep[0] = h * e0;
     ep[1] = h * e1;
     ep[2] = h * e2;
-    eq[0] = h * fma(se, p[0], h * ea0);
-    eq[1] = h * fma(se, p[1], h * ea1);
-    eq[2] = h * fma(se, p[2], h * ea2);
+    eq[0] = fma(hh, ea0, se * hp[0]);
+    eq[1] = fma(hh, ea1, se * hp[1]);
+    eq[2] = fma(hh, ea2, se * hp[2]);
   }
 }
 

@@ -399,8 +399,11 @@ int gx_nbody_integrate(const gx_pot* ext, const double* gm, double eps,
                        int64_t* nsteps, int64_t* naccept, void* workspace, void* stream);
 
 /* Accuracy check of the library's short FP64 primitives (no counterpart in
- * the reference; for tests): out[4i .. 4i+3] = 1/x, 1/sqrt(x) (x > 0),
- * log1p(x) (x > -1) and 1/x by the division routine, at x[i]. */
+ * the reference; for tests): out[9i .. 9i+8] at x = x[i] are 1/x,
+ * 1/sqrt(x) (x > 0), log1p(x) (x > -1), 1/x by the division routine,
+ * sqrt(x), x/6, x^(1/9), x^(1/6) (x >= 0) and x^(1/8) by three reciprocal
+ * square roots (x > 0). */
+#define GX_MATH_CHECK_OUTPUTS 9
 int gx_math_check(const double* x, int64_t n, double* out, void* stream);
 
 #ifdef __cplusplus
