@@ -227,7 +227,9 @@ __device__ __forceinline__ long long next_orbit(const IArgs& A, bool head) {
 }
 
 template <class M, bool PID, class E>
-__global__ void __launch_bounds__(kBlock) k_integrate(DPot P, IArgs A) {
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(E::kWavesPerSimd > 0 ? E::kWavesPerSimd : 1)))
+    k_integrate(DPot P, IArgs A) {
   __shared__ Cold C;
   const int L = threadIdx.x;
   const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -89,18 +89,25 @@ __device__ inline void track_pos(const DTrack& T, double t, double& x,
     z = T.q1[2];
     return;
   }
-  // guess from the mean spacing, then correct to the interval holding t
+  // guess from the mean spacing, then correct to the interval holding t.
+  // The guess is right unless t lies within rounding of a knot: the bounds
+  // of the guessed record are loaded together (one memory round trip, where
+  // checking them one after the other took two), and only a wrong guess
+  // walks to the neighbouring records.
   const long long last = T.n - 2;
   long long k = (long long)floor((t - T.t0) * T.inv_h);
   k = k < 0 ? 0 : (k > last ? last : k);
   const double* r = T.rec + 16 * k;
-  while (k > 0 && r[0] > t) {
-    --k;
-    r -= 16;
-  }
-  while (k < last && r[1] <= t) {
-    ++k;
-    r += 16;
+  const double2 b = *reinterpret_cast<const double2*>(r);
+  if (!((k == 0 || b.x <= t) && (k == last || b.y > t))) {
+    while (k > 0 && r[0] > t) {
+      --k;
+      r -= 16;
+    }
+    while (k < last && r[1] <= t) {
+      ++k;
+      r += 16;
+    }
   }
   const double s = (t - r[0]) * r[2];
   x = fma(s, fma(s, fma(s, r[7], r[6]), r[5]), r[4]);
@@ -606,6 +613,7 @@ struct EvalFixed {
   static constexpr bool kLarge = false;
   static constexpr bool kFrames = false;  // origin-centred table only
   static constexpr bool kInlineFirstStep = false;
+  static constexpr int kWavesPerSimd = 0;  // occupancy target (0: none)
   template <size_t... I>
   __device__ __forceinline__ static void run(const DPot& P, double x, double y,
                                              double z, double& ax, double& ay,
@@ -631,6 +639,7 @@ struct EvalTable {
   static constexpr bool kLarge = FULL;
   static constexpr bool kFrames = FULL;
   static constexpr bool kInlineFirstStep = false;
+  static constexpr int kWavesPerSimd = 0;  // occupancy target (0: none)
   __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
                                              double z, double t, double& ax,
                                              double& ay, double& az) {
@@ -650,6 +659,7 @@ struct EvalMW {
   static constexpr bool kFrames = false;
   // the first step is computed in the integrator (small evaluator)
   static constexpr bool kInlineFirstStep = true;
+  static constexpr int kWavesPerSimd = 2;  // occupancy target (0: none)
   __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
                                              double z, double /*t*/,
                                              double& ax, double& ay,
@@ -695,6 +705,7 @@ struct EvalMilkyWayRotating {
   static constexpr bool kLarge = true;  // rolled stage loop: one copy of the group loop
   static constexpr bool kFrames = true;
   static constexpr bool kInlineFirstStep = false;
+  static constexpr int kWavesPerSimd = 0;  // occupancy target (0: none)
   __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
                                              double z, double t, double& ax,
                                              double& ay, double& az) {
@@ -729,6 +740,7 @@ struct EvalMilkyWayTracked {
   static constexpr bool kLarge = false;
   static constexpr bool kFrames = true;
   static constexpr bool kInlineFirstStep = false;
+  static constexpr int kWavesPerSimd = 2;  // occupancy target (0: none)
   __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
                                              double z, double t, double& ax,
                                              double& ay, double& az) {
