@@ -65,6 +65,22 @@ def test_ctypes_signatures_match_header():
         assert nargs == len(_lib.SIGNATURES[name][1]), name
 
 
+def test_constants_match_header():
+    """Every integer constant of the header (#define and enum members) that
+    the Python mirror names has the header's value, and the mirror names the
+    ones the host code passes through the ABI."""
+    text = HEADER.read_text()
+    consts = {m.group(1): int(m.group(2)) for m in re.finditer(r"^#define\s+(GX_\w+)\s+\(?(-?\d+)\)?", text, flags=re.M)}
+    for body in re.findall(r"enum\s+\w+\s*\{([^}]*)\}", text, flags=re.S):
+        for m in re.finditer(r"(GX_\w+)\s*=\s*(-?\d+)", body):
+            consts[m.group(1)] = int(m.group(2))
+    mirrored = {k: getattr(_abi, k) for k in consts if hasattr(_abi, k)}
+    assert {k: consts[k] for k in mirrored} == mirrored
+    for k in ("GX_MATH_CHECK_OUTPUTS", "GX_MAX_GL_ORDER", "GX_VOGELSBERGER_NFW", "GX_FRAME_TRACK",
+              "GX_DOPRI5", "GX_PID", "GX_NONFINITE", "GX_TS_PER_ORBIT"):
+        assert k in mirrored, k
+
+
 def test_struct_layouts_match_header(tmp_path):
     src = tmp_path / "layout.c"
     src.write_text(
