@@ -139,6 +139,60 @@ def stream_report(steps: int) -> dict:
     }
 
 
+def c1_report(steps: int) -> dict:
+    """The reference's entry point on the correctness configuration:
+    gd.evaluate_orbit over 1,024 Milky Way orbits (R ~ U(4, 20) kpc),
+    0-1000 Myr, 101 save times, (a) Dopri8 at a fixed 0.5 Myr step (2,000
+    steps) and (b) the default Integrator (Dopri8, PID rtol = atol = 1e-7),
+    which galax solves as one ODE over the batch (shared steps).  Inputs are
+    resident on the device; timed with HIP events.  Beside it, the oracle on
+    the same inputs (all orbits, 16 threads) as the CPU baseline and the
+    checker."""
+    import numpy as np
+    import torch
+
+    import galax_amd.dynamics as gd
+    import galax_amd.potential as gp
+    import oracle
+    from galax_amd import workloads
+
+    torch.cuda.set_device(0)
+    pot = gp.MilkyWayPotential()
+    w0, _, _, ts = workloads.config_plain_small()
+    wd = torch.as_tensor(w0, device="cuda")
+    fixed = gd.Integrator(gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.ConstantStepSize(), max_steps=None),
+                          diffeq_kw={"dt0": 0.5})
+    rep = {"report": "gd.evaluate_orbit, 1,024 Milky Way orbits, 0-1000 Myr, 101 save times",
+           "orbits": int(w0.shape[0]), "saves": int(ts.size)}
+    for name, integ, shared in (("fixed_dt_0.5", fixed, False), ("default_pid_1e-7_shared_steps", gd.Integrator(), True)):
+        o = gd.evaluate_orbit(pot, wd, ts, integrator=integ)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(max(steps, 1)):
+            o = gd.evaluate_orbit(pot, wd, ts, integrator=integ)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / max(steps, 1)
+        got = torch.cat([o.q, o.p], dim=-1).cpu().numpy()
+        nst = int(np.asarray(o.nsteps.cpu().numpy() if hasattr(o.nsteps, "cpu") else o.nsteps).max())
+        t_cpu = time.perf_counter()
+        ref, st, ns, _ = oracle.integrate(pot, integ.dynamics_solver, w0, ts[0], ts[-1], ts=ts,
+                                          dt0=integ.diffeq_kw.get("dt0"), shared=shared, nthreads=16)
+        cpu_ms = (time.perf_counter() - t_cpu) * 1e3
+        rel = np.linalg.norm(got - ref, axis=-1) / np.linalg.norm(ref, axis=-1)
+        rep[name] = {
+            "ms": ms,
+            "attempted_steps_per_orbit": nst,
+            "orbit_steps_per_s": w0.shape[0] * nst / (ms * 1e-3),
+            "cpu_oracle_ms": cpu_ms,
+            "cpu_oracle_threads": 16,
+            "oracle_same_steps": bool(int(ns.max()) == nst),
+            "max_rel_err_vs_oracle": float(rel.max()),
+        }
+    return rep
+
+
 def streamfit_report(steps: int) -> dict:
     """Forward + backward of the stream-track loss, timed over `steps`
     repetitions, with the gradient checked against central differences
@@ -229,7 +283,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument(
         "--workload",
-        choices=("uneven", "uniform", "longest", "copy", "surrogate", "streamfit", "stream"),
+        choices=("uneven", "uniform", "longest", "copy", "surrogate", "streamfit", "stream", "c1"),
         default="uneven",
         help="uneven: the benchmark configuration; uniform: the same orbits with every "
         "integration time set to 200 Myr (balanced work: chip throughput); longest: the "
@@ -241,7 +295,9 @@ def main() -> int:
         "released over 3 Gyr, fixed-step Dopri8 h = 1 Myr) w.r.t. six Milky Way parameters by the "
         "discrete adjoint, timed and checked against central differences (prints its own report line); "
         "stream: a 100,000-particle mock stream (50,000 release times over 4 Gyr, Fardal release, "
-        "adaptive Dopri8 rtol 1e-8, the progenitor's Plummer riding its track; prints its own report line)",
+        "adaptive Dopri8 rtol 1e-8, the progenitor's Plummer riding its track; prints its own report line); "
+        "c1: gd.evaluate_orbit on 1,024 Milky Way orbits over 1 Gyr with 101 saves, fixed step 0.5 Myr and the "
+        "default adaptive Integrator (shared steps), beside the oracle (prints its own report line)",
     )
     ap.add_argument("--dump-steps", default=None, help="save per-orbit attempted steps and queue order (.npz)")
     ap.add_argument("--blocks-per-cu", type=int, default=None,
@@ -252,7 +308,7 @@ def main() -> int:
                          "default: the number of passes in flight, at most 4, for the throughput workload)")
     ap.add_argument("--in-flight", type=int, default=None,
                     help="passes (independent batches) in flight on that many streams (1: back to back; "
-                    "default 2 for the throughput workload, 1 for the diagnostic ones)")
+                    "default 6 for the throughput workload, 1 for the diagnostic ones)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -289,6 +345,9 @@ def main() -> int:
         return 0
     if args.workload == "stream":
         print(json.dumps(stream_report(args.steps)), flush=True)
+        return 0
+    if args.workload == "c1":
+        print(json.dumps(c1_report(args.steps)), flush=True)
         return 0
     if args.workload == "streamfit":
         print(json.dumps(streamfit_report(args.steps)), flush=True)
