@@ -90,16 +90,20 @@ __device__ inline void track_pos(const DTrack& T, double t, double& x,
     return;
   }
   // guess from the mean spacing, then correct to the interval holding t.
-  // The guess is right unless t lies within rounding of a knot: the bounds
-  // of the guessed record are loaded together (one memory round trip, where
-  // checking them one after the other took two), and only a wrong guess
-  // walks to the neighbouring records.
+  // The guess is right unless t lies within rounding of a knot, so the
+  // whole 128-byte record is loaded at once (one memory round trip, where
+  // checking the bounds first and loading the coefficients after took two)
+  // and then checked; only a wrong guess walks to the neighbouring records
+  // and loads again.
   const long long last = T.n - 2;
   long long k = (long long)floor((t - T.t0) * T.inv_h);
   k = k < 0 ? 0 : (k > last ? last : k);
-  const double* r = T.rec + 16 * k;
-  const double2 b = *reinterpret_cast<const double2*>(r);
-  if (!((k == 0 || b.x <= t) && (k == last || b.y > t))) {
+  const double2* r2 = reinterpret_cast<const double2*>(T.rec + 16 * k);
+  double2 v[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = r2[i];
+  if (!((k == 0 || v[0].x <= t) && (k == last || v[0].y > t))) {
+    const double* r = T.rec + 16 * k;
     while (k > 0 && r[0] > t) {
       --k;
       r -= 16;
@@ -108,11 +112,15 @@ __device__ inline void track_pos(const DTrack& T, double t, double& x,
       ++k;
       r += 16;
     }
+    r2 = reinterpret_cast<const double2*>(r);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = r2[i];
   }
-  const double s = (t - r[0]) * r[2];
-  x = fma(s, fma(s, fma(s, r[7], r[6]), r[5]), r[4]);
-  y = fma(s, fma(s, fma(s, r[11], r[10]), r[9]), r[8]);
-  z = fma(s, fma(s, fma(s, r[15], r[14]), r[13]), r[12]);
+  // record: t_k, t_k+1, 1/(t_k+1 - t_k), unused, then the x, y, z cubics
+  const double s = (t - v[0].x) * v[1].x;
+  x = fma(s, fma(s, fma(s, v[3].y, v[3].x), v[2].y), v[2].x);
+  y = fma(s, fma(s, fma(s, v[5].y, v[5].x), v[4].y), v[4].x);
+  z = fma(s, fma(s, fma(s, v[7].y, v[7].x), v[6].y), v[6].x);
 }
 
 __device__ inline void frame_delta(const DPot& P, const DComp& c, double t,
