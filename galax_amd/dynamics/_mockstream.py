@@ -148,6 +148,13 @@ class MockStream(dict):
         return np.concatenate([self["lead"].p, self["trail"].p])
 
 
+def release_order(t0, t1):
+    """Work-queue order for a stream's particles: the longest integrations
+    first, so that both arms' earliest releases start on the queue head
+    (ties keep the lead arm first).  Results do not depend on the order."""
+    return np.argsort(-(np.asarray(t1) - np.asarray(t0)), kind="stable")
+
+
 class MockStreamGenerator:
     """Generate a mock stream in `potential` (default integrators: galax's
     legacy Integrator, Dopri8 with PIDController(1e-7, 1e-7))."""
@@ -199,6 +206,7 @@ class MockStreamGenerator:
         res = integrate_orbits(
             pot, w_all, t0_all, np.full(2 * n, t_f), solver=integ.dynamics_solver,
             dt0=integ.diffeq_kw.get("dt0"), max_steps=integ.diffeq_kw.get("max_steps", "default"),
+            order=release_order(t0_all, t_f),
         )
         final = res.out[:, 0, :].cpu().numpy()
         tt = np.full(n, ts[-1])
