@@ -29,6 +29,11 @@ GX_TRIAXIAL_HERNQUIST = 17
 GX_TRIAXIAL_NFW = 18
 GX_LEESUTO_NFW = 19
 GX_VOGELSBERGER_NFW = 20
+GX_HARMONIC_OSCILLATOR = 21
+GX_HENON_HEILES = 22
+GX_HARDCUTOFF_NFW = 23
+GX_GNFW = 24
+GX_GAUSSIAN = 25
 
 GX_MAX_GL_ORDER = 128
 GX_MATH_CHECK_OUTPUTS = 9

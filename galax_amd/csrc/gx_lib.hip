@@ -67,6 +67,22 @@ void gauss_legendre01(int n, std::vector<double>& s, std::vector<double>& w) {
   }
 }
 
+// psi(x), the digamma function, for x > 0: the recurrence psi(x) =
+// psi(x + 1) - 1/x up to x >= 20, then the asymptotic series
+// ln x - 1/(2x) - sum_n B_2n / (2n x^2n) through n = 6 (truncation below
+// 1e-17 there; Abramowitz & Stegun 6.3.5 and 6.3.18).
+double digamma(double x) {
+  double acc = 0.0;
+  while (x < 20.0) {
+    acc -= 1.0 / x;
+    x += 1.0;
+  }
+  const double i2 = 1.0 / (x * x);
+  const double series =
+      i2 * (1.0 / 12 - i2 * (1.0 / 120 - i2 * (1.0 / 252 - i2 * (1.0 / 240 - i2 * (1.0 / 132 - i2 * 691.0 / 32760)))));
+  return acc + std::log(x) - 0.5 / x - series;
+}
+
 // Derived device constants for one component (see DComp in gx_potential.hpp).
 // Components with a table (the triaxial NFW's quadrature nodes) append it to
 // `aux` and keep its offset in p[].
@@ -239,6 +255,41 @@ int derive(const gx_component& in, DComp& out, std::vector<double>& aux) {
       out.p[4] = p[3] * rs;
       break;
     }
+    case GX_HARMONIC_OSCILLATOR:
+      out.p[0] = p[0] * p[0];
+      break;
+    case GX_HENON_HEILES:
+      if (!(p[1] != 0)) return GX_EINVAL;
+      out.p[0] = p[0];
+      out.p[1] = 1.0 / (p[1] * p[1]);
+      break;
+    case GX_HARDCUTOFF_NFW: {
+      const double rs = p[1], rt = p[2];
+      if (!(rs > 0 && rt > 0)) return GX_EINVAL;
+      const double xt = rt / rs;
+      out.p[0] = p[0];
+      out.p[1] = rs;
+      out.p[2] = 1.0 / rs;
+      out.p[3] = p[0] * (std::log1p(xt) - xt / (1.0 + xt));  // G M_NFW(<r_t)
+      out.p[4] = rt;
+      break;
+    }
+    case GX_GNFW: {
+      const double rs = p[1], gamma = p[2];
+      if (!(rs > 0 && gamma >= 0.0 && gamma < 2.0)) return GX_EINVAL;
+      out.p[0] = p[0];
+      out.p[1] = rs;
+      out.p[2] = 1.0 / rs;
+      out.p[3] = 3.0 - gamma;
+      out.p[4] = digamma(3.0 - gamma);
+      break;
+    }
+    case GX_GAUSSIAN:
+      if (!(p[1] > 0)) return GX_EINVAL;
+      out.p[0] = p[0];
+      out.p[1] = p[1];
+      out.p[2] = 1.0 / p[1];
+      break;
     default:
       return GX_EINVAL;
   }

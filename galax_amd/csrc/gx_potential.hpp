@@ -218,6 +218,62 @@ __device__ inline double gammap(double a, double x, double lga) {
   return 1.0 - pre * h;
 }
 
+// B_z(a, 0) = int_0^z u^(a-1) / (1 - u) du (0 <= z < 1, a > 0), the
+// generalized NFW's enclosed mass over m at x = r/r_s, z = x/(1 + x),
+// a = 3 - gamma (nfw/generalized.py:233-330, which takes it as
+// z^a/a 2F1(a, 1; a + 1; z), NIST DLMF 8.17.7).  With w = 1 - z = 1/(1 + x)
+// given separately (no cancellation for large x):
+//   z <= 1/2:  z^a sum_n z^n / (a + n);
+//   z >  1/2:  z^a sum_k (a)_k / k! [psi(k + 1) - psi(a + k) - ln w] w^k,
+// the connection formula of 2F1 in the logarithmic case c = a + b (DLMF
+// 15.8.10, b = 1).  psi_a = psi(a) comes from the host.
+__device__ inline double gnfw_mass(double a, double psi_a, double z, double w) {
+  double sum;
+  if (z <= 0.5) {
+    double term = 1.0;
+    sum = 1.0 / a;
+    for (int n = 1; n < 80; ++n) {
+      term *= z;
+      const double t = term / (a + n);
+      sum += t;
+      if (t < 1e-18 * sum) break;
+    }
+  } else {
+    constexpr double kEulerGamma = 0.57721566490153286061;
+    const double lw = log(w);
+    double c = 1.0, wk = 1.0, psi_k1 = -kEulerGamma, psi_ak = psi_a;
+    sum = psi_k1 - psi_ak - lw;
+    for (int k = 1; k < 120; ++k) {
+      c *= (a + k - 1) / k;
+      wk *= w;
+      psi_k1 += 1.0 / k;
+      psi_ak += 1.0 / (a + k - 1);
+      const double t = c * wk * (psi_k1 - psi_ak - lw);
+      sum += t;
+      if (fabs(t) < 1e-18 * fabs(sum)) break;
+    }
+  }
+  return pow(z, a) * sum;
+}
+
+// The Gaussian sphere's enclosed mass over m_tot at x = r/r_s:
+// erf(x/sqrt2) - sqrt(2/pi) x exp(-x^2/2) (gaussian/base.py:224-235); below
+// x = 1 by its power series sqrt(2/pi) sum_{n>=1} (-1)^(n+1) x^(2n+1) /
+// ((2n+1) 2^(n-1) (n-1)!), where the two terms would cancel.
+__device__ inline double gaussian_mass(double x) {
+  constexpr double kSqrt2OverPi = 0.79788456080286535588;
+  if (x < 1.0) {
+    const double x2 = x * x;
+    double term = x * x2, sum = 0.0;  // x^(2n+1) / (2^(n-1) (n-1)!) at n = 1
+    for (int n = 1; n <= 20; ++n) {
+      sum += ((n & 1) ? term : -term) / (2 * n + 1);
+      term *= x2 / (2.0 * n);
+    }
+    return kSqrt2OverPi * sum;
+  }
+  return erf(x * 0.70710678118654752440) - kSqrt2OverPi * x * exp(-0.5 * x * x);
+}
+
 // --------------------------------------------------------- spherical g
 // For a spherical model Phi(r): a = -(Phi'(r)/r) x.  Returns Phi'(r)/r.
 __device__ __forceinline__ double sph_g(int kind, const double* p, double r2,
@@ -255,6 +311,22 @@ __device__ __forceinline__ double sph_g(int kind, const double* p, double r2,
     }
     case GX_STONE:  // G M(<r) / r^3, M(<r) ~ r_h atan(r/r_h) - r_c atan(r/r_c)
       return p[0] * (p[2] * atan(r / p[2]) - p[1] * atan(r / p[1])) * ir * ir * ir;
+    case GX_HARMONIC_OSCILLATOR:  // Phi'/r = omega^2
+      return p[0];
+    case GX_HARDCUTOFF_NFW: {  // NFW inside r_t, point mass G M(<r_t) outside
+      if (r <= p[4]) {
+        const double L = log1p_fast(r * p[2]);
+        const double u = rcp(r + p[1]);
+        return p[0] * ir * ir * (L * ir - u);
+      }
+      return p[3] * ir * ir * ir;
+    }
+    case GX_GNFW: {  // G m B_z(3 - gamma, 0) / r^3
+      const double w = rcp(1.0 + r * p[2]);
+      return p[0] * gnfw_mass(p[3], p[4], r * p[2] * w, w) * ir * ir * ir;
+    }
+    case GX_GAUSSIAN:  // G m_tot M(<r)/m_tot / r^3
+      return p[0] * gaussian_mass(r * p[2]) * ir * ir * ir;
     default:
       return 0.0;
   }
@@ -460,6 +532,11 @@ __device__ __forceinline__ bool shaped_acc(int kind, const double* p,
   }
   if (kind == GX_TRIAXIAL_NFW) {
     tnfw_acc(p, aux, x, y, z, ax, ay, az);
+    return true;
+  }
+  if (kind == GX_HENON_HEILES) {  // grad (R^2/2 + c (x^2 y - y^3/3)) / t_s^2
+    ax -= p[1] * fma(2.0 * p[0] * x, y, x);
+    ay -= p[1] * fma(p[0], (x - y) * (x + y), y);
     return true;
   }
   if (kind >= GX_TRIAXIAL_HERNQUIST && kind <= GX_VOGELSBERGER_NFW) {
@@ -915,6 +992,25 @@ __device__ inline double phi_model(const DComp& c, const double* aux,
       return -p[0] * ((rh * atan(r / rh) - rc * atan(r / rc)) / r +
                       0.5 * log((r2 + rh * rh) / (r2 + rc * rc)));
     }
+    case GX_HARMONIC_OSCILLATOR:
+      return 0.5 * p[0] * (x * x + y * y + z * z);
+    case GX_HENON_HEILES:
+      return p[1] * (0.5 * (x * x + y * y) + p[0] * (x * x * y - y * y * y / 3.0));
+    case GX_HARDCUTOFF_NFW: {  // NFW + G m/(r_s + r_t) inside r_t, -G M(<r_t)/r outside
+      const double r = sqrt(x * x + y * y + z * z + kTiny);
+      if (r <= p[4]) return -p[0] * log1p(r * p[2]) / r + p[0] / (p[1] + p[4]);
+      return -p[3] / r;
+    }
+    case GX_GNFW: {  // -G m (B_z(a, 0)/r + (1 - z^(2-gamma)) / ((2 - gamma) r_s))
+      const double r = sqrt(x * x + y * y + z * z + kTiny);
+      const double w = 1.0 / (1.0 + r * p[2]), zz = r * p[2] * w;
+      const double b = p[3] - 1.0;
+      return -p[0] * (gnfw_mass(p[3], p[4], zz, w) / r + (1.0 - pow(zz, b)) * p[2] / b);
+    }
+    case GX_GAUSSIAN: {
+      const double r = sqrt(x * x + y * y + z * z + kTiny);
+      return -p[0] * erf(r * p[2] * 0.70710678118654752440) / r;
+    }
     case GX_POWERLAW_CUTOFF: {
       const double r2 = x * x + y * y + z * z + kTiny, r = sqrt(r2), s2 = r2 * p[2];
       return -p[0] * (gammap(p[1], s2, p[3]) / r + p[5] * (1.0 - gammap(p[4], s2, p[6])));
@@ -997,7 +1093,11 @@ __device__ inline void hess_model(const DComp& c, const double* aux,
     case GX_LOGARITHMIC:
     case GX_POWERLAW_CUTOFF:
     case GX_BURKERT:
-    case GX_STONE: {
+    case GX_STONE:
+    case GX_HARMONIC_OSCILLATOR:
+    case GX_HARDCUTOFF_NFW:
+    case GX_GNFW:
+    case GX_GAUSSIAN: {
       // H = g I + (Phi'' - g) xx^T / r^2, g = Phi'/r
       const double r2 = x * x + y * y + z * z;
       const double rr = r2 + kTiny;
@@ -1045,6 +1145,30 @@ __device__ inline void hess_model(const DComp& c, const double* aux,
         const double D = 1.0 / (p[1] + rr);
         d1 = GM * r * D;  // GM slot holds v_c^2
         d2 = GM * (p[1] - rr) * D * D;
+      } else if (c.kind == GX_HARMONIC_OSCILLATOR) {
+        d1 = GM * r;  // GM slot holds omega^2
+        d2 = GM;
+      } else if (c.kind == GX_HARDCUTOFF_NFW && r > p[4]) {
+        d1 = p[3] / rr;
+        d2 = -2.0 * p[3] / (rr * r);
+      } else if (c.kind == GX_HARDCUTOFF_NFW) {
+        const double rs = p[1], L = log1p(r * p[2]);
+        const double u = 1.0 / (r + rs);
+        d1 = GM * (L / rr - u / r);
+        d2 = GM * (u / rr - 2.0 * L / (rr * r) + (2.0 * r + rs) * u * u / rr);
+      } else if (c.kind == GX_GNFW) {
+        // M(<r) = m B(x), dB/dr = x^(a-1) (1 + x)^(-a) / r_s
+        const double X = r * p[2], w = 1.0 / (1.0 + X), a = p[3];
+        const double B = gnfw_mass(a, p[4], X * w, w);
+        const double dB = pow(X, a - 1.0) * pow(w, a) * p[2];
+        d1 = GM * B / rr;
+        d2 = GM * (dB / rr - 2.0 * B / (rr * r));
+      } else if (c.kind == GX_GAUSSIAN) {
+        // dM/dr = m_tot sqrt(2/pi) x^2 exp(-x^2/2) / r_s
+        const double X = r * p[2], M = gaussian_mass(X);
+        const double dM = 0.79788456080286535588 * X * X * exp(-0.5 * X * X) * p[2];
+        d1 = GM * M / rr;
+        d2 = GM * (dM / rr - 2.0 * M / (rr * r));
       } else {
         const double d = rr + p[1];
         const double id = 1.0 / sqrt(d);
@@ -1109,6 +1233,16 @@ __device__ inline void hess_model(const DComp& c, const double* aux,
           H[3 * a + b] += p[0] * (F * HW[3 * a + b] + G * X[a] * gW[b] +
                                   gW[a] * G * X[b] + W * HF);
         }
+      break;
+    }
+    case GX_HENON_HEILES: {
+      // (R^2/2 + c (x^2 y - y^3/3)) / t_s^2: Phi_xx = 1 + 2 c y,
+      // Phi_xy = 2 c x, Phi_yy = 1 - 2 c y (over t_s^2), z-free
+      const double hxy = p[1] * 2.0 * p[0] * x;
+      H[0] += p[1] * (1.0 + 2.0 * p[0] * y);
+      H[1] += hxy;
+      H[3] += hxy;
+      H[4] += p[1] * (1.0 - 2.0 * p[0] * y);
       break;
     }
     case GX_KUZMIN:

@@ -19,6 +19,11 @@ from ._builtin import (
     TriaxialNFWPotential,
     LeeSutoTriaxialNFWPotential,
     Vogelsberger08TriaxialNFWPotential,
+    HarmonicOscillatorPotential,
+    HenonHeilesPotential,
+    HardCutoffNFWPotential,
+    gNFWPotential,
+    GaussianPotential,
     KeplerPotential,
     MiyamotoNagaiPotential,
     MonariEtAl2016BarPotential,
@@ -82,6 +87,11 @@ __all__ = [
     "TriaxialNFWPotential",
     "LeeSutoTriaxialNFWPotential",
     "Vogelsberger08TriaxialNFWPotential",
+    "HarmonicOscillatorPotential",
+    "HenonHeilesPotential",
+    "HardCutoffNFWPotential",
+    "gNFWPotential",
+    "GaussianPotential",
     "PowerLawCutoffPotential",
     "BovyMWPotential2014",
     "MN3ExponentialPotential",

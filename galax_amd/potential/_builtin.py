@@ -538,3 +538,95 @@ class Vogelsberger08TriaxialNFWPotential(AbstractSinglePotential):
     def __repr__(self):
         return (f"Vogelsberger08TriaxialNFWPotential(m={self.m!r}, r_s={self.r_s!r}, "
                 f"q1={self.q1!r}, a_r={self.a_r!r})")
+
+
+class HarmonicOscillatorPotential(AbstractSinglePotential):
+    """Phi = omega^2 |q|^2 / 2 (example.py:17-100); omega in 1/Myr."""
+
+    kind = _abi.GX_HARMONIC_OSCILLATOR
+
+    def __init__(self, omega, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.omega = _p(omega, "1 / Myr")
+
+    def _params(self):
+        return (self.omega,)
+
+    def __repr__(self):
+        return f"HarmonicOscillatorPotential(omega={self.omega!r})"
+
+
+class HenonHeilesPotential(AbstractSinglePotential):
+    """Henon & Heiles (1964) in specific-energy units: Phi = (R^2/2 +
+    coeff (x^2 y - y^3/3)) / timescale^2 (example.py:103-176); coeff in
+    1/kpc, timescale in Myr."""
+
+    kind = _abi.GX_HENON_HEILES
+
+    def __init__(self, coeff, timescale, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.coeff = _p(coeff, "1 / kpc")
+        self.timescale = _p(timescale, "Myr")
+        if self.timescale == 0:
+            raise ValueError("timescale must be nonzero")
+
+    def _params(self):
+        return (self.coeff, self.timescale)
+
+    def __repr__(self):
+        return f"HenonHeilesPotential(coeff={self.coeff!r}, timescale={self.timescale!r})"
+
+
+class HardCutoffNFWPotential(AbstractSinglePotential):
+    """NFW sharply truncated at r_t: Phi_NFW + G m / (r_s + r_t) inside,
+    -G M_NFW(<r_t) / r outside (nfw/truncated.py:37-330)."""
+
+    kind = _abi.GX_HARDCUTOFF_NFW
+
+    def __init__(self, m, r_s, r_t, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m = _p(m, "Msun")
+        self.r_s = _p(r_s, "kpc")
+        self.r_t = _p(r_t, "kpc")
+        if not (self.r_s > 0 and self.r_t > 0):
+            raise ValueError("need r_s > 0 and r_t > 0")
+
+    def _params(self):
+        return (G * self.m, self.r_s, self.r_t)
+
+    def __repr__(self):
+        return f"HardCutoffNFWPotential(m={self.m!r}, r_s={self.r_s!r}, r_t={self.r_t!r})"
+
+
+class gNFWPotential(AbstractSinglePotential):  # noqa: N801 (the reference's name)
+    """Generalized NFW, rho ~ x^-gamma (1 + x)^(gamma - 3), gamma in [0, 2):
+    M(<r) = m B_z(3 - gamma, 0), z = x/(1 + x), and
+    Phi = -G (M(<r)/r + m/r_s B_{1/(1+x)}(1, 2 - gamma))
+    (nfw/generalized.py:30-330)."""
+
+    kind = _abi.GX_GNFW
+
+    def __init__(self, m, r_s, gamma, *, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m = _p(m, "Msun")
+        self.r_s = _p(r_s, "kpc")
+        self.gamma = _p(gamma)
+        if not (0.0 <= self.gamma < 2.0 and self.r_s > 0):
+            raise ValueError("need 0 <= gamma < 2 and r_s > 0")
+
+    def _params(self):
+        return (G * self.m, self.r_s, self.gamma)
+
+    def __repr__(self):
+        return f"gNFWPotential(m={self.m!r}, r_s={self.r_s!r}, gamma={self.gamma!r})"
+
+
+class GaussianPotential(_MassScale):
+    """Spherical Gaussian density: Phi = -G m_tot erf(r / (sqrt2 r_s)) / r
+    (gaussian/base.py:40-259)."""
+
+    kind = _abi.GX_GAUSSIAN

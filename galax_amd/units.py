@@ -57,6 +57,10 @@ _FACTORS = {
     "km / (kpc s)": KMS,
     "km/s/kpc": KMS,
     "1/Gyr": 1e-3,
+    # wavenumber -> 1/kpc
+    "1/kpc": 1.0,
+    "1 / kpc": 1.0,
+    "1/pc": 1e3,
     # acceleration -> kpc/Myr^2
     "kpc/Myr2": 1.0,
     "kpc / Myr2": 1.0,

@@ -67,6 +67,17 @@ extern "C" {
  *                                                            nfw/leesuto.py:29-164
  *   GX_VOGELSBERGER_NFW p0 = G*m, p1 = r_s, p2 = q1 (q2^2 = 3 - q1^2),
  *                      p3 = a_r                              nfw/vogelsburger08.py:27-110
+ *   GX_HARMONIC_OSCILLATOR p0 = omega [1/Myr] (Phi = omega^2 r^2 / 2)
+ *                                                            example.py:17-100
+ *   GX_HENON_HEILES    p0 = coeff [1/kpc], p1 = timescale [Myr]
+ *                      (Phi = (R^2/2 + coeff (x^2 y - y^3/3)) / t_s^2)
+ *                                                            example.py:103-176
+ *   GX_HARDCUTOFF_NFW  p0 = G*m, p1 = r_s, p2 = r_t > 0 (NFW inside r_t,
+ *                      point mass M_NFW(<r_t) outside)       nfw/truncated.py:37-330
+ *   GX_GNFW            p0 = G*m, p1 = r_s, p2 = gamma in [0, 2) (inner slope)
+ *                                                            nfw/generalized.py:30-330
+ *   GX_GAUSSIAN        p0 = G*m_tot, p1 = r_s (Phi = -G m erf(r/(sqrt2 r_s))/r)
+ *                                                            gaussian/base.py:40-259
  */
 enum gx_kind {
   GX_NULL = 0,
@@ -89,7 +100,12 @@ enum gx_kind {
   GX_TRIAXIAL_HERNQUIST = 17,
   GX_TRIAXIAL_NFW = 18,
   GX_LEESUTO_NFW = 19,
-  GX_VOGELSBERGER_NFW = 20
+  GX_VOGELSBERGER_NFW = 20,
+  GX_HARMONIC_OSCILLATOR = 21,
+  GX_HENON_HEILES = 22,
+  GX_HARDCUTOFF_NFW = 23,
+  GX_GNFW = 24,
+  GX_GAUSSIAN = 25
 };
 
 /* Where the component sits (potential/_src/xfm/translate.py:29-199 —

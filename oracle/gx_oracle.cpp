@@ -257,6 +257,63 @@ J<O> jcos(const J<O>& a) {
   double c = std::cos(a.v), s = std::sin(a.v);
   return chain(a, c, -s, -c);
 }
+// B(x) = B_z(a, 0) = int_0^z u^(a-1) / (1 - u) du with z = x / (1 + x): the
+// generalized NFW's enclosed mass over m (nfw/generalized.py:233-330).  By
+// double-exponential (tanh-sinh) quadrature in long double, u = z (1 +
+// tanh(pi/2 sinh s)) / 2, step 1/64 over |s| <= 4 (a different route from
+// the device's series, so that each checks the other).
+double gnfw_mass_q(double a, double x) {
+  const long double z = (long double)x / (1.0L + x);
+  const long double h = 1.0L / 64, pi2 = 1.5707963267948966192L;
+  long double sum = 0.0L;
+  for (int k = -256; k <= 256; ++k) {
+    const long double s = k * h;
+    const long double sh = std::sinh(s), ch = std::cosh(s);
+    const long double th = std::tanh(pi2 * sh);
+    const long double cw = std::cosh(pi2 * sh);
+    const long double u = 0.5L * z * (1.0L + th);
+    const long double du = 0.5L * z * pi2 * ch / (cw * cw);
+    // 1 - u = (1 - z) + z (1 - th)/2, with 1 - th = 2 / (1 + exp(pi sh))
+    const long double omu = 1.0L / (1.0L + x) + z / (1.0L + std::exp(2.0L * pi2 * sh));
+    if (u <= 0.0L) continue;
+    sum += du * std::pow(u, (long double)a - 1.0L) / omu;
+  }
+  return (double)(h * sum);
+}
+// B(X) for a jet X, with B'(x) = x^(a-1) (1 + x)^(-a) and
+// B''(x) = x^(a-2) (1 + x)^(-a-1) (a - 1 - x).
+template <int O>
+J<O> jgnfw_mass(double a, const J<O>& X) {
+  const double x = X.v;
+  const double d1 = std::pow(x, a - 1.0) * std::pow(1.0 + x, -a);
+  const double d2 = std::pow(x, a - 2.0) * std::pow(1.0 + x, -a - 1.0) * (a - 1.0 - x);
+  return chain(X, gnfw_mass_q(a, x), d1, d2);
+}
+template <int O>
+J<O> jpow(const J<O>& a, double b) {
+  return chain(a, std::pow(a.v, b), b * std::pow(a.v, b - 1.0), b * (b - 1.0) * std::pow(a.v, b - 2.0));
+}
+template <int O>
+J<O> jerf(const J<O>& a) {
+  const double d1 = 1.1283791670955125739 * std::exp(-a.v * a.v);
+  return chain(a, std::erf(a.v), d1, -2.0 * a.v * d1);
+}
+// erf(q^(1/2)) / q^(1/2) for small q as (2/sqrt(pi)) S(q), S(q) =
+// sum_m t_m / (2m + 1) with t_m = (-q)^m / m!; then S'(q) = -sum_m t_m /
+// (2m + 3) and S''(q) = sum_m t_m / (2m + 5) (term-by-term derivatives).
+template <int O>
+J<O> jerf_over_sqrt(const J<O>& q) {
+  double f = 0, d1 = 0, d2 = 0, t = 1.0;
+  for (int m = 0; m < 40; ++m) {
+    f += t / (2 * m + 1);
+    d1 -= t / (2 * m + 3);
+    d2 += t / (2 * m + 5);
+    t *= -q.v / (m + 1);
+  }
+  const double c = 1.1283791670955125739;
+  return chain(q, c * f, c * d1, c * d2);
+}
+
 template <int O>
 J<O> jatan2(const J<O>& y, const J<O>& x) {
   // F(y, x) = atan2(y, x); Fy = x/r2, Fx = -y/r2,
@@ -445,6 +502,41 @@ J<O> phi_model(const gx_component& c, const J<O>& x, const J<O>& y,
       J<O> at = (rh * jatan(r / rh) - rc * jatan(r / rc)) / r;
       J<O> lg = 0.5 * jlog((sq(r) + rh * rh) / (sq(r) + rc * rc));
       return (-2.0 * p[0] / (M_PI * (rh - rc))) * (at + lg);
+    }
+    case GX_HARMONIC_OSCILLATOR: {  // omega^2 r^2 / 2 (example.py:75-85)
+      return (0.5 * p[0] * p[0]) * (sq(x) + sq(y) + sq(z));
+    }
+    case GX_HENON_HEILES: {  // (R^2/2 + c (x^2 y - y^3/3)) / t_s^2 (example.py:159-176)
+      const double c = p[0], its2 = 1.0 / (p[1] * p[1]);
+      return its2 * (0.5 * (sq(x) + sq(y)) + c * (sq(x) * y - (1.0 / 3.0) * (sq(y) * y)));
+    }
+    case GX_HARDCUTOFF_NFW: {
+      // r <= r_t: Phi_NFW + G m / (r_s + r_t); r > r_t: -G m mu(r_t/r_s) / r,
+      // mu(x) = log(1 + x) - x / (1 + x)  (nfw/truncated.py:255-330)
+      const double rs = p[1], rt = p[2];
+      J<O> r = r_spherical(x, y, z);
+      if (r.v <= rt) return (-p[0] / rs) * jlog1p(r / rs) / (r / rs) + p[0] / (rs + rt);
+      const double xt = rt / rs;
+      return (-p[0] * (std::log1p(xt) - xt / (1.0 + xt))) / r;
+    }
+    case GX_GNFW: {
+      // -G m (B(x)/r + (1 - z^(2-gamma)) / ((2 - gamma) r_s)), z = x/(1 + x),
+      // B(x) = B_z(3 - gamma, 0)  (nfw/generalized.py:30-330)
+      const double rs = p[1], g = p[2], a = 3.0 - g, b = 2.0 - g;
+      J<O> r = r_spherical(x, y, z);
+      J<O> X = r / rs;
+      J<O> zz = X / (1.0 + X);
+      return (-p[0]) * (jgnfw_mass(a, X) / r + (1.0 / (b * rs)) * (1.0 - jpow(zz, b)));
+    }
+    case GX_GAUSSIAN: {
+      // -G m erf(r / (sqrt2 r_s)) / r  (gaussian/base.py:76-95,240-259); for
+      // r < sqrt2 r_s through the series of erf(X)/X in X^2, which keeps the
+      // derivatives free of cancellation near the centre
+      const double rs = p[1], k = 1.0 / (std::sqrt(2.0) * rs);
+      J<O> r = r_spherical(x, y, z);
+      J<O> X = k * r;
+      if (X.v < 1.0) return (-p[0] * k) * jerf_over_sqrt(sq(X));
+      return (-p[0]) * jerf(X) / r;
     }
     case GX_TRIAXIAL_HERNQUIST: {
       // -G m / (r' + r_s), r' = sqrt(x^2 + (y/q1)^2 + (z/q2)^2 + tiny)

@@ -95,6 +95,11 @@ def zoo():
         "triaxial_nfw": gp.TriaxialNFWPotential(m=5e11, r_s=12.0, q1=0.9, q2=0.8),
         "leesuto_nfw": gp.LeeSutoTriaxialNFWPotential(m=6e11, r_s=15.0, a1=1.0, a2=0.9, a3=0.8),
         "vogelsberger_nfw": gp.Vogelsberger08TriaxialNFWPotential(m=5e11, r_s=12.0, q1=0.9, a_r=1.5),
+        "harmonic": gp.HarmonicOscillatorPotential(omega=0.05),
+        "henon_heiles": gp.HenonHeilesPotential(coeff=0.02, timescale=20.0),
+        "hardcutoff_nfw": gp.HardCutoffNFWPotential(m=1e12, r_s=15.0, r_t=40.0),
+        "gnfw": gp.gNFWPotential(m=5e11, r_s=12.0, gamma=0.7),
+        "gaussian": gp.GaussianPotential(m_tot=1e11, r_s=5.0),
         "offset_nfw": gp.TranslatedPotential(gp.NFWPotential(m=1e11, r_s=3.0), [1.0, -2.0, 0.5]),
         "offset_bar": gp.TranslatedPotential(bar, [0.5, 0.0, 0.0]),
         "track_plummer": gp.TranslatedPotential(gp.PlummerPotential(m_tot=1e9, r_s=0.3), track),

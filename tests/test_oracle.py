@@ -130,6 +130,76 @@ def test_extra_known_answers(golden):
             np.testing.assert_allclose(oracle.evaluate(pot, "acceleration", x, e["t"]), e["acceleration"], atol=5.1e-9)
 
 
+def test_more_builtin_models_known_answers():
+    """The reference's doctest values for the harmonic oscillator and
+    Henon-Heiles (example.py:55-56, 136-137), the generalized NFW
+    (nfw/generalized.py:85-105), the hard-cutoff NFW's enclosed masses
+    (nfw/truncated.py:130-151) and the Gaussian's (gaussian/base.py:131-136),
+    the enclosed masses read off the gradient as |grad Phi| r^2 / G."""
+    phi = lambda pot, q: float(oracle.evaluate(pot, "potential", np.array([q], float))[0])
+    mass = lambda pot, r: np.linalg.norm(oracle.evaluate(pot, "gradient", np.array([[r, 0.0, 0.0]]))[0]) * r * r / G
+    assert phi(gp.HarmonicOscillatorPotential(omega=1.0), [1, 0, 0]) == pytest.approx(0.5, abs=1e-15)
+    assert phi(gp.HenonHeilesPotential(coeff=1.0, timescale=1.0), [1, 0, 0]) == pytest.approx(0.5, abs=1e-15)
+    g1 = gp.gNFWPotential(m=1e12, r_s=1, gamma=1)
+    assert phi(g1, [8, 0, 0]) == pytest.approx(-1.23552744, abs=5.1e-9)
+    assert phi(g1, [0, 0, 0]) == pytest.approx(-4.49850215, abs=5.1e-9)
+    assert phi(gp.gNFWPotential(m=1e12, r_s=1, gamma=0.5), [8, 0, 0]) == pytest.approx(-1.09363913, abs=5.1e-9)
+    hc = gp.HardCutoffNFWPotential(m=1e11, r_s=15, r_t=20)
+    for r, m in ((20.0, 2.75869289e10), (25.0, 2.75869289e10), (10.0, 1.10825624e10)):
+        assert mass(hc, r) == pytest.approx(m, rel=5e-9)
+    assert mass(gp.GaussianPotential(m_tot=1e11, r_s=15), 10.0) == pytest.approx(6.90842509e9, rel=5e-9)
+
+
+def test_more_builtin_models_pinned_by_closed_forms():
+    """gNFW at gamma = 1 is NFW and at gamma = 0 has the closed-form mass
+    m (-log(1 - z) - z - z^2/2), z = x/(1 + x), which pins the quadrature of
+    B_z(a, 0) at a = 2 and a = 3; the hard-cutoff NFW is NFW plus a constant
+    inside r_t and a point mass outside; the Gaussian tends to a point mass;
+    and trace H = 4 pi G rho for the closed-form densities."""
+    rng = np.random.default_rng(5)
+    d = rng.normal(size=(40, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    x = d * np.exp(rng.uniform(np.log(0.05), np.log(80.0), 40))[:, None]
+    r = np.linalg.norm(x, axis=1)
+    nfw = gp.NFWPotential(m=3e11, r_s=7.0)
+    g1 = gp.gNFWPotential(m=3e11, r_s=7.0, gamma=1.0)
+    for what, tol in (("potential", 1e-14), ("gradient", 1e-13), ("hessian", 1e-12)):
+        a, b = oracle.evaluate(g1, what, x), oracle.evaluate(nfw, what, x)
+        assert np.max(np.abs(a - b) / np.max(np.abs(b))) < tol, what
+    g0 = gp.gNFWPotential(m=3e11, r_s=7.0, gamma=0.0)
+    z = (r / 7.0) / (1 + r / 7.0)
+    # -log(1 - z) - z - z^2/2 = sum_{n>=3} z^n / n (the series where the
+    # closed form cancels)
+    series = sum(z**n / n for n in range(3, 120))
+    m0 = 3e11 * np.where(z < 0.5, series, -np.log1p(-z) - z - z * z / 2)
+    np.testing.assert_allclose(np.linalg.norm(oracle.evaluate(g0, "gradient", x), axis=1) * r * r / G, m0, rtol=1e-12)
+    hc = gp.HardCutoffNFWPotential(m=3e11, r_s=7.0, r_t=20.0)
+    inside = r <= 20.0
+    dphi = oracle.evaluate(hc, "potential", x[inside]) - oracle.evaluate(nfw, "potential", x[inside])
+    np.testing.assert_allclose(dphi, G * 3e11 / 27.0, rtol=1e-12)
+    np.testing.assert_allclose(oracle.evaluate(hc, "gradient", x[inside]), oracle.evaluate(nfw, "gradient", x[inside]),
+                               rtol=1e-13, atol=0)
+    mt = 3e11 * (np.log1p(20 / 7) - (20 / 7) / (1 + 20 / 7))
+    np.testing.assert_allclose(oracle.evaluate(hc, "potential", x[~inside]), -G * mt / r[~inside], rtol=1e-13)
+    ga = gp.GaussianPotential(m_tot=1e11, r_s=2.0)
+    far = np.array([[300.0, 0, 0]])
+    assert oracle.evaluate(ga, "potential", far)[0] == pytest.approx(-G * 1e11 / 300.0, rel=1e-15)
+    # Poisson: trace H = 4 pi G rho, to 1e-9 of rho plus the round-off of
+    # a sum of Hessian entries that cancel where rho is small (1e-13 of |H|)
+    def poisson(pot, q, rho, rtol=1e-9):
+        H = oracle.evaluate(pot, "hessian", q)
+        tr = H[:, 0, 0] + H[:, 1, 1] + H[:, 2, 2]
+        want = 4 * np.pi * G * rho
+        bound = rtol * np.abs(want) + 1e-13 * np.linalg.norm(H.reshape(len(q), 9), axis=1)
+        assert np.all(np.abs(tr - want) <= bound), np.max(np.abs(tr - want) / bound)
+    s = r / 2.0
+    poisson(ga, x, 1e11 / ((2 * np.pi) ** 1.5 * 8.0) * np.exp(-s * s / 2))
+    s = r / 7.0
+    poisson(gp.gNFWPotential(m=3e11, r_s=7.0, gamma=0.6), x, 3e11 / (4 * np.pi * 343.0) * s**-0.6 * (1 + s) ** (0.6 - 3))
+    poisson(gp.HarmonicOscillatorPotential(omega=0.05), x, 3 * 0.05**2 / (4 * np.pi * G), rtol=1e-14)
+    poisson(gp.HenonHeilesPotential(coeff=0.02, timescale=20.0), x, 2 / 400.0 / (4 * np.pi * G), rtol=1e-13)
+
+
 def test_gradient_matches_finite_differences():
     from _pots import zoo
 
