@@ -101,6 +101,39 @@ def test_field_scalar_and_batched_time_shapes():
 
 
 # ------------------------------------------------------------ integration
+@pytest.mark.parametrize("name", ["harmonic", "henon_heiles", "hardcutoff_nfw", "gnfw", "gaussian"])
+def test_fixed_step_parity_more_models(name):
+    """256 bound orbits in each of the later kinds, Dopri8 fixed step 0.5
+    Myr over 200 Myr with 9 saves, against the oracle (the field differs by
+    round-off only, so the trajectories agree to round-off growth)."""
+    pot = zoo()[name]
+    rng = np.random.default_rng(21)
+    n = 256
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    q = d * rng.uniform(2.0, 12.0, n)[:, None]
+    if name == "henon_heiles":  # bound orbits: well inside the escape energy
+        q *= 0.4
+        q[:, 2] *= 0.2
+    g = np.linalg.norm(oracle.evaluate(pot, "gradient", q), axis=1)
+    vc = np.sqrt(g * np.linalg.norm(q, axis=1))  # |v| near circular
+    v = rng.normal(size=(n, 3))
+    v -= np.sum(v * d, axis=1, keepdims=True) * d
+    v *= (0.8 * vc / np.linalg.norm(v, axis=1))[:, None]
+    w0 = np.concatenate([q, v], axis=1)
+    ts = np.linspace(0.0, 200.0, 9)
+    solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.ConstantStepSize(), max_steps=None)
+    say("fixed-step parity", name, n)
+    res = gd.integrate_orbits(pot, w0, 0.0, 200.0, ts=ts, solver=solver, dt0=0.5)
+    out = res.out.cpu().numpy()
+    ref, st, ns, _ = oracle.integrate(pot, solver, w0, 0.0, 200.0, ts=ts, dt0=0.5)
+    assert np.all(res.status.cpu().numpy() == 0) and np.all(st == 0)
+    np.testing.assert_array_equal(res.nsteps.cpu().numpy(), ns)
+    e = _rel(out, ref)
+    print(name, "max rel", e.max(), flush=True)
+    assert e.max() < 1e-12
+
+
 def test_fixed_step_plain_small_parity():
     """1,024 MW orbits, Dopri8 fixed step 0.5 Myr, 0..1 Gyr, on-grid saves."""
     w0, t0, t1, ts = workloads.config_plain_small()
