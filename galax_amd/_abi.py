@@ -34,6 +34,7 @@ GX_HENON_HEILES = 22
 GX_HARDCUTOFF_NFW = 23
 GX_GNFW = 24
 GX_GAUSSIAN = 25
+GX_TRIAXIAL_GAUSSIAN = 26
 
 GX_MAX_GL_ORDER = 128
 GX_MATH_CHECK_OUTPUTS = 9

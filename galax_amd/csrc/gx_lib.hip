@@ -290,6 +290,30 @@ int derive(const gx_component& in, DComp& out, std::vector<double>& aux) {
       out.p[1] = p[1];
       out.p[2] = 1.0 / p[1];
       break;
+    case GX_TRIAXIAL_GAUSSIAN: {
+      // the triaxial NFW's node table [s^2, s^2/A1, s^2/A2, w/sqrt(A1 A2)]
+      const double rs = p[1], q1 = p[2], q2 = p[3];
+      const int n = (int)p[4];
+      if (!(rs > 0 && q1 > 0 && q2 > 0) || p[4] != (double)n || n < 1 ||
+          n > GX_MAX_GL_ORDER)
+        return GX_EINVAL;
+      std::vector<double> sk, wk;
+      gauss_legendre01(n, sk, wk);
+      out.p[0] = 2.0 * p[0] / (std::sqrt(2.0 * M_PI) * rs * rs * rs);
+      out.p[1] = 1.0 / (rs * rs);
+      out.p[2] = (double)aux.size();
+      out.p[3] = n;
+      out.p[4] = rs * rs;
+      for (int k = 0; k < n; ++k) {
+        const double s2 = sk[k] * sk[k];
+        const double A1 = 1.0 + (q1 * q1 - 1.0) * s2, A2 = 1.0 + (q2 * q2 - 1.0) * s2;
+        aux.push_back(s2);
+        aux.push_back(s2 / A1);
+        aux.push_back(s2 / A2);
+        aux.push_back(wk[k] / std::sqrt(A1 * A2));
+      }
+      break;
+    }
     default:
       return GX_EINVAL;
   }

@@ -534,6 +534,23 @@ __device__ __forceinline__ bool shaped_acc(int kind, const double* p,
     tnfw_acc(p, aux, x, y, z, ax, ay, az);
     return true;
   }
+  if (kind == GX_TRIAXIAL_GAUSSIAN) {
+    // grad Phi_i = P x_i sum_k W_k C_ki exp(-xi_k^2/2), xi_k^2 = sum_i C_ki x_i^2 / r_s^2
+    // (gaussian/triaxial.py:132-221 differentiated under the integral)
+    const double* tab = aux + (long long)p[2];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int k = 0; k < (int)p[3]; ++k) {
+      const double c0 = tab[4 * k], c1 = tab[4 * k + 1], c2 = tab[4 * k + 2];
+      const double e = tab[4 * k + 3] * exp(-0.5 * p[1] * (c0 * x * x + c1 * y * y + c2 * z * z));
+      sx = fma(c0, e, sx);
+      sy = fma(c1, e, sy);
+      sz = fma(c2, e, sz);
+    }
+    ax -= p[0] * sx * x;
+    ay -= p[0] * sy * y;
+    az -= p[0] * sz * z;
+    return true;
+  }
   if (kind == GX_HENON_HEILES) {  // grad (R^2/2 + c (x^2 y - y^3/3)) / t_s^2
     ax -= p[1] * fma(2.0 * p[0] * x, y, x);
     ay -= p[1] * fma(p[0], (x - y) * (x + y), y);
@@ -994,6 +1011,14 @@ __device__ inline double phi_model(const DComp& c, const double* aux,
     }
     case GX_HARMONIC_OSCILLATOR:
       return 0.5 * p[0] * (x * x + y * y + z * z);
+    case GX_TRIAXIAL_GAUSSIAN: {
+      const double* tab = aux + (long long)p[2];
+      double sum = 0.0;
+      for (int k = 0; k < (int)p[3]; ++k)
+        sum += tab[4 * k + 3] *
+               exp(-0.5 * p[1] * (tab[4 * k] * x * x + tab[4 * k + 1] * y * y + tab[4 * k + 2] * z * z));
+      return -p[0] * p[4] * sum;
+    }
     case GX_HENON_HEILES:
       return p[1] * (0.5 * (x * x + y * y) + p[0] * (x * x * y - y * y * y / 3.0));
     case GX_HARDCUTOFF_NFW: {  // NFW + G m/(r_s + r_t) inside r_t, -G M(<r_t)/r outside
@@ -1233,6 +1258,18 @@ __device__ inline void hess_model(const DComp& c, const double* aux,
           H[3 * a + b] += p[0] * (F * HW[3 * a + b] + G * X[a] * gW[b] +
                                   gW[a] * G * X[b] + W * HF);
         }
+      break;
+    }
+    case GX_TRIAXIAL_GAUSSIAN: {
+      // H_ij = P sum_k W_k e_k (delta_ij C_ki - C_ki C_kj x_i x_j / r_s^2)
+      const double* tab = aux + (long long)p[2];
+      for (int k = 0; k < (int)p[3]; ++k) {
+        const double C[3] = {tab[4 * k], tab[4 * k + 1], tab[4 * k + 2]};
+        const double e = p[0] * tab[4 * k + 3] * exp(-0.5 * p[1] * (C[0] * x * x + C[1] * y * y + C[2] * z * z));
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b)
+            H[3 * a + b] += e * ((a == b ? C[a] : 0.0) - C[a] * C[b] * X[a] * X[b] * p[1]);
+      }
       break;
     }
     case GX_HENON_HEILES: {

@@ -24,6 +24,8 @@ from ._builtin import (
     HardCutoffNFWPotential,
     gNFWPotential,
     GaussianPotential,
+    TriaxialGaussianPotential,
+    AxisymmetricGaussianPotential,
     KeplerPotential,
     MiyamotoNagaiPotential,
     MonariEtAl2016BarPotential,
@@ -92,6 +94,8 @@ __all__ = [
     "HardCutoffNFWPotential",
     "gNFWPotential",
     "GaussianPotential",
+    "TriaxialGaussianPotential",
+    "AxisymmetricGaussianPotential",
     "PowerLawCutoffPotential",
     "BovyMWPotential2014",
     "MN3ExponentialPotential",

@@ -630,3 +630,42 @@ class GaussianPotential(_MassScale):
     (gaussian/base.py:40-259)."""
 
     kind = _abi.GX_GAUSSIAN
+
+
+class TriaxialGaussianPotential(AbstractSinglePotential):
+    """Gaussian density on ellipsoids, rho = m_tot / (q1 q2 (2 pi)^(3/2)
+    r_s^3) exp(-xi^2/2), r_s^2 xi^2 = x^2 + y^2/q1^2 + z^2/q2^2; Phi by the
+    reference's Gauss-Legendre shell integral (gaussian/triaxial.py:26-240)."""
+
+    kind = _abi.GX_TRIAXIAL_GAUSSIAN
+
+    def __init__(self, m_tot, r_s, q1=1.0, q2=1.0, *, integration_order=50, units="galactic"):
+        if units != "galactic":
+            raise ValueError("only the galactic unit system is supported")
+        self.m_tot = _p(m_tot, "Msun")
+        self.r_s = _p(r_s, "kpc")
+        self.q1, self.q2 = _p(q1), _p(q2)
+        self.integration_order = int(integration_order)
+        if not (1 <= self.integration_order <= _abi.GX_MAX_GL_ORDER):
+            raise ValueError(f"integration_order must lie in 1..{_abi.GX_MAX_GL_ORDER}")
+        if not (self.r_s > 0 and self.q1 > 0 and self.q2 > 0):
+            raise ValueError("need r_s, q1, q2 > 0")
+
+    def _params(self):
+        return (G * self.m_tot, self.r_s, self.q1, self.q2, float(self.integration_order))
+
+    def __repr__(self):
+        return (f"TriaxialGaussianPotential(m_tot={self.m_tot!r}, r_s={self.r_s!r}, q1={self.q1!r}, "
+                f"q2={self.q2!r}, integration_order={self.integration_order})")
+
+
+class AxisymmetricGaussianPotential(TriaxialGaussianPotential):
+    """The triaxial Gaussian with q1 = 1: rho ~ exp(-(R^2 + z^2/q2^2) /
+    (2 r_s^2)) (gaussian/axisymmetric.py:29-206)."""
+
+    def __init__(self, m_tot, r_s, q2=1.0, *, integration_order=50, units="galactic"):
+        super().__init__(m_tot, r_s, 1.0, q2, integration_order=integration_order, units=units)
+
+    def __repr__(self):
+        return (f"AxisymmetricGaussianPotential(m_tot={self.m_tot!r}, r_s={self.r_s!r}, q2={self.q2!r}, "
+                f"integration_order={self.integration_order})")

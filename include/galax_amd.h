@@ -78,6 +78,10 @@ extern "C" {
  *                                                            nfw/generalized.py:30-330
  *   GX_GAUSSIAN        p0 = G*m_tot, p1 = r_s (Phi = -G m erf(r/(sqrt2 r_s))/r)
  *                                                            gaussian/base.py:40-259
+ *   GX_TRIAXIAL_GAUSSIAN p0 = G*m_tot, p1 = r_s, p2 = q1, p3 = q2 (density
+ *                      axis ratios; q1 = 1 is the axisymmetric model),
+ *                      p4 = Gauss-Legendre order (1..128, default 50)
+ *                                      gaussian/triaxial.py:26-240, axisymmetric.py
  */
 enum gx_kind {
   GX_NULL = 0,
@@ -105,7 +109,8 @@ enum gx_kind {
   GX_HENON_HEILES = 22,
   GX_HARDCUTOFF_NFW = 23,
   GX_GNFW = 24,
-  GX_GAUSSIAN = 25
+  GX_GAUSSIAN = 25,
+  GX_TRIAXIAL_GAUSSIAN = 26
 };
 
 /* Where the component sits (potential/_src/xfm/translate.py:29-199 —

@@ -294,6 +294,11 @@ J<O> jpow(const J<O>& a, double b) {
   return chain(a, std::pow(a.v, b), b * std::pow(a.v, b - 1.0), b * (b - 1.0) * std::pow(a.v, b - 2.0));
 }
 template <int O>
+J<O> jexp(const J<O>& a) {
+  const double e = std::exp(a.v);
+  return chain(a, e, e, e);
+}
+template <int O>
 J<O> jerf(const J<O>& a) {
   const double d1 = 1.1283791670955125739 * std::exp(-a.v * a.v);
   return chain(a, std::erf(a.v), d1, -2.0 * a.v * d1);
@@ -527,6 +532,22 @@ J<O> phi_model(const gx_component& c, const J<O>& x, const J<O>& y,
       J<O> X = r / rs;
       J<O> zz = X / (1.0 + X);
       return (-p[0]) * (jgnfw_mass(a, X) / r + (1.0 / (b * rs)) * (1.0 - jpow(zz, b)));
+    }
+    case GX_TRIAXIAL_GAUSSIAN: {
+      // -2 pi G rho0 r_s^2 q1 q2 int_0^1 2 exp(-xi^2/2) / sqrt(A1 A2) ds,
+      // r_s^2 xi^2 = s^2 (x^2 + y^2/A1 + z^2/A2), A_i = 1 + (q_i^2 - 1) s^2,
+      // rho0 = m / (q1 q2 (2 pi)^(3/2) r_s^3)  (gaussian/triaxial.py:86-221)
+      const double rs = p[1], q1sq = p[2] * p[2], q2sq = p[3] * p[3];
+      const std::vector<double>& sw = gauss_legendre01((int)p[4]);
+      J<O> sum = jconst<O>(0.0);
+      for (int k = 0; k < (int)p[4]; ++k) {
+        const double s2 = sw[2 * k] * sw[2 * k];
+        const double A1 = 1.0 + (q1sq - 1.0) * s2, A2 = 1.0 + (q2sq - 1.0) * s2;
+        J<O> xi2 = (s2 / (rs * rs)) * (sq(x) + sq(y) / A1 + sq(z) / A2);
+        sum = sum + (sw[2 * k + 1] / std::sqrt(A1 * A2)) * (2.0 * jexp(-0.5 * xi2));
+      }
+      // 2 pi G rho0 r_s^2 q1 q2 = G m / (sqrt(2 pi) r_s)
+      return (-p[0] / (std::sqrt(2.0 * M_PI) * rs)) * sum;
     }
     case GX_GAUSSIAN: {
       // -G m erf(r / (sqrt2 r_s)) / r  (gaussian/base.py:76-95,240-259); for

@@ -100,6 +100,7 @@ def zoo():
         "hardcutoff_nfw": gp.HardCutoffNFWPotential(m=1e12, r_s=15.0, r_t=40.0),
         "gnfw": gp.gNFWPotential(m=5e11, r_s=12.0, gamma=0.7),
         "gaussian": gp.GaussianPotential(m_tot=1e11, r_s=5.0),
+        "triaxial_gaussian": gp.TriaxialGaussianPotential(m_tot=2e11, r_s=6.0, q1=0.9, q2=0.6),
         "offset_nfw": gp.TranslatedPotential(gp.NFWPotential(m=1e11, r_s=3.0), [1.0, -2.0, 0.5]),
         "offset_bar": gp.TranslatedPotential(bar, [0.5, 0.0, 0.0]),
         "track_plummer": gp.TranslatedPotential(gp.PlummerPotential(m_tot=1e9, r_s=0.3), track),

@@ -101,7 +101,7 @@ def test_field_scalar_and_batched_time_shapes():
 
 
 # ------------------------------------------------------------ integration
-@pytest.mark.parametrize("name", ["harmonic", "henon_heiles", "hardcutoff_nfw", "gnfw", "gaussian"])
+@pytest.mark.parametrize("name", ["harmonic", "henon_heiles", "hardcutoff_nfw", "gnfw", "gaussian", "triaxial_gaussian"])
 def test_fixed_step_parity_more_models(name):
     """256 bound orbits in each of the later kinds, Dopri8 fixed step 0.5
     Myr over 200 Myr with 9 saves, against the oracle (the field differs by

@@ -21,7 +21,8 @@ from _pots import from_spec, ic, solver_from
                                   "isochrone", "jaffe", "logarithmic", "lmj09_logarithmic", "longmurali_bar",
                                   "mn3_exponential", "mn3_sech2", "milky_way_2022", "lm10", "powerlaw_cutoff",
                                   "bovy_mw_2014", "burkert", "kuzmin", "satoh", "stone", "triaxial_hernquist",
-                                  "triaxial_nfw", "leesuto_nfw", "vogelsberger_nfw"])
+                                  "triaxial_nfw", "leesuto_nfw", "vogelsberger_nfw", "gaussian",
+                                  "axisymmetric_gaussian", "triaxial_gaussian"])
 def test_potential_known_answers(golden, name):
     e = next(x for x in golden["potentials"] if x["name"] == name)
     pot = from_spec(e["class"], e["params"])
