@@ -226,8 +226,10 @@ __device__ inline double gammap(double a, double x, double lga) {
 //   z <= 1/2:  z^a sum_n z^n / (a + n);
 //   z >  1/2:  z^a sum_k (a)_k / k! [psi(k + 1) - psi(a + k) - ln w] w^k,
 // the connection formula of 2F1 in the logarithmic case c = a + b (DLMF
-// 15.8.10, b = 1).  psi_a = psi(a) comes from the host.
-__device__ inline double gnfw_mass(double a, double psi_a, double z, double w) {
+// 15.8.10, b = 1).  psi_a = psi(a) comes from the host.  Kept out of line
+// (as is gaussian_mass): inlined, the series and library calls raised the
+// generic table kernels above 256 VGPRs, halving their occupancy.
+__device__ __noinline__ double gnfw_mass(double a, double psi_a, double z, double w) {
   double sum;
   if (z <= 0.5) {
     double term = 1.0;
@@ -260,7 +262,7 @@ __device__ inline double gnfw_mass(double a, double psi_a, double z, double w) {
 // erf(x/sqrt2) - sqrt(2/pi) x exp(-x^2/2) (gaussian/base.py:224-235); below
 // x = 1 by its power series sqrt(2/pi) sum_{n>=1} (-1)^(n+1) x^(2n+1) /
 // ((2n+1) 2^(n-1) (n-1)!), where the two terms would cancel.
-__device__ inline double gaussian_mass(double x) {
+__device__ __noinline__ double gaussian_mass(double x) {
   constexpr double kSqrt2OverPi = 0.79788456080286535588;
   if (x < 1.0) {
     const double x2 = x * x;
