@@ -188,3 +188,48 @@ def test_numpy_gradient_matches_oracle():
     np.testing.assert_allclose(
         workloads.mw_gradient_np(x), oracle.evaluate(gp.MilkyWayPotential(), "gradient", x), rtol=1e-12
     )
+
+
+def _kernel_resources(tmp_path):
+    """Kernel metadata (VGPRs, spills, scratch) of every kernel in the built
+    library's gfx950 code objects: the .hip_fatbin section holds one offload
+    bundle per source file."""
+    tools = Path("/opt/rocm/llvm/bin")
+    need = [tools / "llvm-objcopy", tools / "clang-offload-bundler", tools / "llvm-readelf"]
+    if not _lib.LIB_PATH.exists() or not all(t.exists() for t in need):
+        pytest.skip("library or ROCm LLVM tools missing")
+    fat = tmp_path / "fat.bin"
+    subprocess.run([str(need[0]), f"--dump-section=.hip_fatbin={fat}", str(_lib.LIB_PATH), str(tmp_path / "stripped.so")],
+                   check=True, capture_output=True)
+    data = fat.read_bytes()
+    magic = b"__CLANG_OFFLOAD_BUNDLE__"
+    starts = [m.start() for m in re.finditer(re.escape(magic), data)]
+    out = {}
+    for i, s in enumerate(starts):
+        part = tmp_path / f"b{i}.bin"
+        part.write_bytes(data[s:(starts[i + 1] if i + 1 < len(starts) else len(data))])
+        dev = tmp_path / f"b{i}.o"
+        subprocess.run([str(need[1]), "--unbundle", "--type=o", f"--input={part}",
+                        "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--output={dev}"], check=True, capture_output=True)
+        notes = subprocess.run([str(need[2]), "--notes", str(dev)], check=True, capture_output=True, text=True).stdout
+        for blk in notes.split("- .agpr_count")[1:]:
+            name = re.search(r"\.name:\s+(\S+)", blk).group(1)
+            get = lambda k: int(re.search(rf"\.{k}:\s+(\d+)", blk).group(1))
+            out[name] = (get("vgpr_count"), get("vgpr_spill_count"), get("private_segment_fixed_size"))
+    return out
+
+
+def test_integrator_kernels_keep_two_waves_per_simd(tmp_path):
+    """The Milky Way, tracked-Plummer and frame-less table integrators stay
+    within 256 VGPRs (two waves per SIMD; above it a SIMD holds one wave of
+    them and the throughput halves): a guard against additions to the field
+    code that the compiler inlines into every kernel."""
+    res = _kernel_resources(tmp_path)
+    want = {"EvalMW": 0, "EvalMilkyWayTracked": None, "EvalTableILb0E": None}
+    for key, max_spill in want.items():
+        ks = {n: v for n, v in res.items() if "k_integrate" in n and "Dopri8" in n and key in n}
+        assert ks, key
+        for n, (vgpr, spill, _) in ks.items():
+            assert vgpr <= 256, (n, vgpr)
+            if max_spill is not None:
+                assert spill <= max_spill, (n, spill)
