@@ -278,7 +278,8 @@ IN_FLIGHT = 6
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=48,
+                    help="timed passes (48: the pipeline fill and drain are a small share of the timed region)")
     ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--n", type=int, default=1_048_576)
     ap.add_argument("--saves", type=int, default=64)
