@@ -42,6 +42,8 @@ def lib() -> ctypes.CDLL:
         L.orc_gauss_legendre.argtypes = [ctypes.c_int, _vp, _vp]
         L.orc_tableau.restype = ctypes.c_int
         L.orc_tableau.argtypes = [ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_int)]
+        L.orc_dense_weights.restype = ctypes.c_int
+        L.orc_dense_weights.argtypes = [ctypes.c_int, ctypes.c_double, _vp]
         L.orc_rk_step.restype = ctypes.c_int
         L.orc_rk_step.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, ctypes.c_double, ctypes.c_int64, _vp, _vp]
         L.orc_integrate.restype = ctypes.c_int
@@ -128,6 +130,15 @@ def tableau(method: int):
     order = ctypes.c_int()
     s = lib().orc_tableau(method, c.ctypes.data, a.ctypes.data, b.ctypes.data, be.ctypes.data, ctypes.byref(order))
     return c[:s], a[: s * s].reshape(s, s), b[:s], be[:s], order.value
+
+
+def dense_weights(method: int, th: float) -> np.ndarray:
+    """Weights w_i(th) of the dense output, y(t0 + th h) = y0 + sum_i w_i k_i,
+    over the step's increments (Dopri8: 13 stages and the first-same-as-last
+    increment h f(t1, y1); Dopri5: its 7 stages)."""
+    w = np.zeros(16)
+    ns = lib().orc_dense_weights(int(method), float(th), w.ctypes.data)
+    return w[:ns]
 
 
 def rk_step(pot, method: int, t: float, y, h: float):

@@ -32,9 +32,9 @@
 //   * both batch semantics: independent per-orbit solves
 //     (orbit/solver.py:854-1039) and one shared solve over a (B,6) state
 //     (orbit/solver.py:774-812).
-// Interior save times are filled by a step of the method from the last
-// accepted step start to the save time (the GPU library does the same);
-// diffrax's own Dopri8 interpolant is not restated (parity unpinned there).
+// Interior save times are served by the accepted step's dense output, as
+// diffrax's SaveAt(ts) does (the Dopri8 and Dopri5 interpolants are restated
+// at the Tableau below).
 
 #include "../include/galax_amd.h"
 
@@ -731,7 +731,18 @@ struct Tableau {
   double a[13][13] = {};
   double b[13] = {};
   double be[13] = {};  // b_sol - b_embedded
+  // Dense output over s + 1 increments: k_1..k_s and the first-same-as-last
+  // increment k_{s+1} = h f(t1, y1) (Dopri5's own last stage is already at
+  // y1, so for it k_{s+1} equals k_s up to rounding).
+  int dense = 0;        // kDensePoly or kDenseQuartic
+  int dn = 0;           // polynomial degree of the weights in theta
+  double dw[14][6] = {};  // kDensePoly: b_i(th) = th * polyval(dw[i], th),
+                          // coefficients from th^(dn-1) down to th^0
+  double cmid[7] = {};    // kDenseQuartic: y(1/2) = y0 + sum cmid_i k_i
 };
+
+constexpr int kDensePoly = 0;
+constexpr int kDenseQuartic = 1;
 
 Tableau make_dopri8() {
   // Prince & Dormand (1981), RK8(7)13M.
@@ -842,6 +853,44 @@ Tableau make_dopri8() {
     T.b[i] = b[i];
     T.be[i] = b[i] - bh[i];
   }
+  // diffrax 0.7.0's Dopri8 interpolant (_solver/dopri8.py,
+  // _Dopri8Interpolation; third-party, not vendored; used through
+  // SaveAt(ts) at legacy/integrator.py:227-230 and orbit/solver.py:794):
+  // y(t0 + th h) = y0 + sum_{i=1}^{14} b_i(th) k_i with b_i(th) =
+  // th * (quintic in th), over the 13 stages and the first-same-as-last
+  // increment.  Stages 2-5 carry no weight.  Row 1 is not tabulated: the
+  // weights sum to th (the interpolant reproduces y' = const exactly), which
+  // fixes it from the others.  The reference's Kepler integrate_field
+  // doctest (dynamics/_src/solver.py:342-365) pins the off-grid values.
+  T.dense = kDensePoly;
+  T.dn = 6;
+  const double w[9][6] = {
+      {-39.6107919852202505218, 116.4422149550342161651, -121.4999627731334642623,
+       52.2273532792945524050, -7.6142658045872677172, 0},
+      {20.3761213808791436958, -67.1451318825957197185, 83.1721004639847717481,
+       -46.8919164181093621583, 10.7281392630428866124, 0},
+      {7.3347098826795362023, -16.5672243527496524646, 9.5724507555993664382,
+       -0.1890893225010595467, 0.5526637063753648783, 0},
+      {32.8801774352459155182, -89.9916014847245016028, 87.8406057677205645007,
+       -35.7075975946222072821, 4.2186562625665153803, 0},
+      {-10.1588990526426760954, 22.6237489648532849093, -17.4152107770762969005,
+       6.2736448083240352160, -0.6627209125361597559, 0},
+      {-12.5401268098782561200, 32.2362340167355370113, -28.5903289514790976966,
+       10.3160881272450748458, -1.2636789001135462218, 0},
+      {29.5553001484516038033, -82.1020315488359848644, 81.6630950584341412934,
+       -34.7650769866611817349, 5.4106037898590422230, 0},
+      {-41.7923486424390588923, 116.2662185791119533462, -114.9375291377009418170,
+       47.7457971078225540396, -7.0321379067945741781, 0},
+      {20.3006925822100825485, -53.9020777466385396792, 50.2558364226176017553,
+       -19.0082099341608028453, 2.3537586759714983486, 0}};
+  // rows of stages 6..14 (indices 5..13)
+  for (int r = 0; r < 9; ++r)
+    for (int k = 0; k < 6; ++k) T.dw[5 + r][k] = w[r][k];
+  for (int k = 0; k < 6; ++k) {
+    double acc = (k == 5) ? 1.0 : 0.0;
+    for (int i = 5; i < 14; ++i) acc -= T.dw[i][k];
+    T.dw[0][k] = acc;
+  }
   return T;
 }
 
@@ -887,6 +936,21 @@ Tableau make_dopri5() {
     T.b[i] = b[i];
     T.be[i] = be[i];
   }
+  // diffrax 0.7.0's Dopri5 interpolant (_solver/dopri5.py,
+  // _Dopri5Interpolation, a FourthOrderPolynomialInterpolation from
+  // _local_interpolation.py; third-party): the quartic through y0, y1, the
+  // end-point slopes k_1, k_7 and Shampine's (1986) midpoint value
+  // y0 + sum cmid_i k_i.
+  T.dense = kDenseQuartic;
+  T.dn = 4;
+  const double cm[7] = {6025192743.0 / 30085553152.0 / 2,
+                        0,
+                        51252292925.0 / 65400821598.0 / 2,
+                        -2691868925.0 / 45128329728.0 / 2,
+                        187940372067.0 / 1594534317056.0 / 2,
+                        -1776094331.0 / 19743644256.0 / 2,
+                        11237099.0 / 235043384.0 / 2};
+  for (int i = 0; i < 7; ++i) T.cmid[i] = cm[i];
   return T;
 }
 
@@ -953,8 +1017,10 @@ struct Field {
 // One explicit RK step of size h from (tau, y[6*B]) for B orbits sharing the
 // step: stage increments k = h * f, Y_i = y + sum_j a_ij k_j,
 // y1 = y + sum_i b_i k_i, err = sum_i be_i k_i.
+// kall, if given, receives the increments k_1..k_s ([s][B][6]).
 void rk_step(const Tableau& T, const Field& F, double tau, const double* y,
-             double h, int64_t B, double* y1, double* err) {
+             double h, int64_t B, double* y1, double* err,
+             std::vector<double>* kall = nullptr) {
   const int s = T.s;
   std::vector<double> k((size_t)s * 6 * B), Y(6 * B), fb(6 * B);
   for (int i = 0; i < s; ++i) {
@@ -982,6 +1048,45 @@ void rk_step(const Tableau& T, const Field& F, double tau, const double* y,
       y1[o * 6 + m] = y[o * 6 + m] + sb;
       if (err) err[o * 6 + m] = se;
     }
+  if (kall) kall->swap(k);
+}
+
+// Number of increments the dense output reads: Dopri8 adds the
+// first-same-as-last increment h f(t1, y1) as a 14th.
+int dense_slots(const Tableau& T) {
+  return T.dense == kDensePoly ? T.s + 1 : T.s;
+}
+
+// The step's dense output at th in [0, 1]: y(tau + th h) for B states,
+// from y0, y1 and the increments k[dense_slots][B][6].
+void dense_eval(const Tableau& T, const double* y0, const double* y1,
+                const double* k, int64_t B, double th, double* out) {
+  const int ns = dense_slots(T);
+  const size_t n6 = 6 * (size_t)B;
+  if (T.dense == kDensePoly) {
+    double w[14];
+    for (int i = 0; i < ns; ++i) {
+      double acc = 0.0;
+      for (int d = 0; d < T.dn; ++d) acc = acc * th + T.dw[i][d];
+      w[i] = acc * th;
+    }
+    for (size_t j = 0; j < n6; ++j) {
+      double acc = 0.0;
+      for (int i = 0; i < ns; ++i) acc += w[i] * k[i * n6 + j];
+      out[j] = y0[j] + acc;
+    }
+    return;
+  }
+  // quartic through y0, y1, the end slopes and the midpoint value
+  for (size_t j = 0; j < n6; ++j) {
+    double ymid = y0[j];
+    for (int i = 0; i < T.s; ++i) ymid += T.cmid[i] * k[i * n6 + j];
+    const double f0 = k[j], f1 = k[(size_t)(T.s - 1) * n6 + j];
+    const double a = 2 * (f1 - f0) - 8 * (y1[j] + y0[j]) + 16 * ymid;
+    const double b = 5 * f0 - 3 * f1 + 18 * y0[j] + 14 * y1[j] - 32 * ymid;
+    const double c = f1 - 4 * f0 - 11 * y0[j] - 5 * y1[j] + 16 * ymid;
+    out[j] = (((a * th + b) * th + c) * th + f0) * th + y0[j];
+  }
 }
 
 double rms(const double* x, int64_t n) {
@@ -1046,7 +1151,7 @@ SolveOut solve(const Pot& P, const gx_solver_cfg& cfg, const double* y0,
       for (int m = 0; m < 6; ++m)
         out[((size_t)o * T + s) * 6 + m] = y[o * 6 + m];
   };
-  std::vector<double> y(y0, y0 + n6), y1(n6), err(n6), ys(n6);
+  std::vector<double> y(y0, y0 + n6), y1(n6), err(n6), ys(n6), kst, f1(n6);
   if (t0 == t1) {
     for (int s = 0; s < T; ++s) write(s, y.data());
     return R;
@@ -1066,6 +1171,13 @@ SolveOut solve(const Pot& P, const gx_solver_cfg& cfg, const double* y0,
   bool at_dtmin = false;
   if (cfg.dt0 > 0) {
     h0 = cfg.dt0;
+  } else if (pid && dtmin > 0) {
+    // With dtmin set and no dt0 the first step is dtmin: the reference's
+    // integrate_field doctest (dynamics/_src/solver.py:342-365, default
+    // solver :226-232, dtmin 0.05 forced) reaches its printed end point only
+    // from that first step; the automatic rule (0.130 Myr there) misses it
+    // by 8.5e-6.
+    h0 = dtmin;
   } else {
     h0 = initial_step(F, tau, y.data(), n6, cfg.rtol, cfg.atol, Tb.order);
   }
@@ -1093,7 +1205,7 @@ SolveOut solve(const Pot& P, const gx_solver_cfg& cfg, const double* y0,
       break;
     }
     const double h = tnext - tau;
-    rk_step(Tb, F, tau, y.data(), h, B, y1.data(), err.data());
+    rk_step(Tb, F, tau, y.data(), h, B, y1.data(), err.data(), &kst);
     R.nsteps++;
     if (!all_finite(y1.data(), n6)) {
       R.status = GX_NONFINITE;
@@ -1136,13 +1248,20 @@ SolveOut solve(const Pot& P, const gx_solver_cfg& cfg, const double* y0,
       next_tau1 = tnext + cfg.dt0;
     }
     if (keep) {
-      // saves in (tau, tnext]
+      // saves in (tau, tnext]: the step end, or the step's dense output
+      bool have_fsal = Tb.dense != kDensePoly;
       while (si < T && dir * ts[si] <= tnext) {
         double ts_tau = dir * ts[si];
         if (ts_tau == tnext) {
           write(si, y1.data());
         } else {
-          rk_step(Tb, F, tau, y.data(), ts_tau - tau, B, ys.data(), nullptr);
+          if (!have_fsal) {
+            F.all(tnext, y1.data(), B, f1.data());
+            kst.resize((size_t)(Tb.s + 1) * n6);
+            for (int64_t j = 0; j < n6; ++j) kst[(size_t)Tb.s * n6 + j] = h * f1[j];
+            have_fsal = true;
+          }
+          dense_eval(Tb, y.data(), y1.data(), kst.data(), B, (ts_tau - tau) / h, ys.data());
           write(si, ys.data());
         }
         ++si;
@@ -1229,6 +1348,24 @@ int orc_tableau(int method, double* c, double* a, double* b, double* be,
   }
   *order = T.order;
   return T.s;
+}
+
+// The dense output's weights at th: out(th) - y0 = sum_i w_i k_i, one weight
+// per increment (dense_slots of them, returned), read off dense_eval with
+// one unit increment at a time.
+int orc_dense_weights(int method, double th, double* w) {
+  const Tableau& T = tableau(method);
+  const int ns = dense_slots(T);
+  std::vector<double> k((size_t)ns * 6, 0.0), y0(6, 0.0), y1(6), o(6);
+  for (int i = 0; i < ns; ++i) {
+    std::fill(k.begin(), k.end(), 0.0);
+    for (int m = 0; m < 6; ++m) k[(size_t)i * 6 + m] = 1.0;
+    const double bi = i < T.s ? T.b[i] : 0.0;
+    for (int m = 0; m < 6; ++m) y1[m] = bi;
+    dense_eval(T, y0.data(), y1.data(), k.data(), 1, th, o.data());
+    w[i] = o[0];
+  }
+  return ns;
 }
 
 // One step of size h from (t, y[6B]) for B orbits (the OrbitSolver.step

@@ -338,20 +338,36 @@ def test_three_decimal_doctests(golden, key):
     np.testing.assert_allclose(out[0, 0, 3:], e["p"], atol=5e-4)
 
 
-def test_kepler_integrate_field_doctest_tolerance(golden):
-    """integrate_field(Kepler, rtol=1e-7, dtmin=0.05, 100 saves): the end point
-    agrees to 1e-5 (10% of the method's own global error there); interior
-    saves use diffrax's interpolant in the reference, ours a method step, so
-    they agree to its interpolation error (~3e-6)."""
+def test_kepler_integrate_field_doctest(golden):
+    """integrate_field(Kepler) with galax's default solver (Dopri8, PID 1e-7,
+    dtmin 0.05 forced; dynamics/_src/solver.py:226-232,342-365): ys[1] and
+    ys[-2] lie inside steps and come from diffrax's Dopri8 interpolant, ys[-1]
+    is the end point.  All three are held to the printed eight decimals; the
+    end point is reached only when the first step is dtmin."""
     e = golden["integrator"]["integrate_field_kepler_pid1e7"]
     pot = from_spec(e["potential"]["class"], e["potential"]["params"])
     w0, _ = ic(e)
     a, b, n = e["ts_linspace"]
     ts = np.linspace(a, b, n)
-    out, st, *_ = oracle.integrate(pot, solver_from(e["solver"]), w0, 0.0, 1000.0, ts=ts)
+    out, st, ns, na = oracle.integrate(pot, solver_from(e["solver"]), w0, 0.0, 1000.0, ts=ts)
+    assert st[0] == 0
     for k in ("1", "-2", "-1"):
-        np.testing.assert_allclose(out[0, int(k), :3], e["ys_q"][k], atol=1e-5)
-        np.testing.assert_allclose(out[0, int(k), 3:], e["ys_p"][k], atol=1e-6)
+        np.testing.assert_allclose(out[0, int(k), :3], e["ys_q"][k], atol=5.5e-9)
+        np.testing.assert_allclose(out[0, int(k), 3:], e["ys_p"][k], atol=5.5e-9)
+
+
+def test_first_step_is_dtmin_when_dtmin_is_set(golden):
+    """The automatic first step (0.130 Myr here) is not used when dtmin is set:
+    a solve started from it misses the doctest's end point by ~8e-6."""
+    e = golden["integrator"]["integrate_field_kepler_pid1e7"]
+    pot = from_spec(e["potential"]["class"], e["potential"]["params"])
+    w0, _ = ic(e)
+    s = solver_from(e["solver"])
+    auto, *_ = oracle.integrate(pot, s, w0, 0.0, 1000.0, dt0=0.1303729587544578)
+    dflt, *_ = oracle.integrate(pot, s, w0, 0.0, 1000.0)
+    forced, *_ = oracle.integrate(pot, s, w0, 0.0, 1000.0, dt0=0.05)
+    np.testing.assert_array_equal(dflt, forced)
+    assert np.max(np.abs(auto[0, 0, :3] - e["ys_q"]["-1"])) > 1e-6
 
 
 # ------------------------------------------------------------ closed forms
@@ -438,19 +454,88 @@ def test_against_scipy_dop853():
 
 
 # ---------------------------------------------------------- save semantics
-def test_interior_save_is_a_step_from_the_last_accepted_point():
+def test_saves_on_the_step_grid_are_step_ends():
     pot = gp.HernquistPotential(m_tot=1e12, r_s=5.0)
     y0 = np.array([[10.0, 0.0, 0.0, 0.0, 0.2, 0.01]])
     s = OrbitSolver(Dopri8(), stepsize_controller=ConstantStepSize(), max_steps=None)
     out, *_ = oracle.integrate(pot, s, y0, 0.0, 3.0, ts=np.array([0.0, 0.3, 1.0, 2.7, 3.0]), dt0=1.0)
-    y1, _ = oracle.rk_step(pot, _abi.GX_DOPRI8, 0.0, y0, 0.3)
-    np.testing.assert_array_equal(out[0, 1], y1[0])
     ya, _ = oracle.rk_step(pot, _abi.GX_DOPRI8, 0.0, y0, 1.0)
     np.testing.assert_array_equal(out[0, 2], ya[0])
     yb, _ = oracle.rk_step(pot, _abi.GX_DOPRI8, 1.0, ya, 1.0)
-    yc, _ = oracle.rk_step(pot, _abi.GX_DOPRI8, 2.0, yb, 2.7 - 2.0)
-    np.testing.assert_array_equal(out[0, 3], yc[0])
+    yc, _ = oracle.rk_step(pot, _abi.GX_DOPRI8, 2.0, yb, 1.0)
+    np.testing.assert_array_equal(out[0, 4], yc[0])
     np.testing.assert_array_equal(out[0, 0], y0[0])
+
+
+def _stage_increments(pot, method, y0, h):
+    """The step's increments k_i = h f(Y_i) in Python from the exported
+    tableau (Dopri8 also gets h f(y1) as its 14th)."""
+    c, A, b, be, order = oracle.tableau(method)
+    f = lambda y: np.concatenate([y[3:], oracle.evaluate(pot, "acceleration", y[None, :3])[0]])
+    k = []
+    for i in range(len(c)):
+        k.append(h * f(y0 + sum(A[i, j] * k[j] for j in range(i))))
+    y1 = y0 + sum(b[i] * k[i] for i in range(len(c)))
+    if method == _abi.GX_DOPRI8:
+        k.append(h * f(y1))
+    return np.array(k)
+
+
+@pytest.mark.parametrize("method", [_abi.GX_DOPRI8, _abi.GX_DOPRI5])
+def test_interior_saves_are_the_steps_dense_output(method):
+    pot = gp.HernquistPotential(m_tot=1e12, r_s=5.0)
+    y0 = np.array([10.0, 0.0, 0.0, 0.0, 0.2, 0.01])
+    meth = Dopri8() if method == _abi.GX_DOPRI8 else Dopri5()
+    s = OrbitSolver(meth, stepsize_controller=ConstantStepSize(), max_steps=None)
+    out, *_ = oracle.integrate(pot, s, y0[None], 0.0, 2.0, ts=np.array([0.3, 1.0, 1.7]), dt0=1.0)
+    k0 = _stage_increments(pot, method, y0, 1.0)
+    np.testing.assert_allclose(out[0, 0], y0 + oracle.dense_weights(method, 0.3) @ k0, rtol=0, atol=1e-14)
+    k1 = _stage_increments(pot, method, out[0, 1], 1.0)
+    np.testing.assert_allclose(out[0, 2], out[0, 1] + oracle.dense_weights(method, 0.7) @ k1, rtol=0, atol=1e-14)
+
+
+def test_dopri8_dense_weights():
+    """diffrax's Dopri8 interpolant: zero at th = 0, the step's weights b at
+    th = 1 (the 14th, first-same-as-last weight vanishes there), weights that
+    sum to th, and the quadrature conditions sum_i w_i c_i^(k-1) = th^k / k
+    through k = 5 at every th (c_14 = 1)."""
+    c, A, b, be, order = oracle.tableau(_abi.GX_DOPRI8)
+    c14 = np.append(c, 1.0)
+    assert np.all(oracle.dense_weights(_abi.GX_DOPRI8, 0.0) == 0.0)
+    np.testing.assert_allclose(oracle.dense_weights(_abi.GX_DOPRI8, 1.0), np.append(b, 0.0), rtol=0, atol=2e-12)
+    for th in np.linspace(0.05, 0.95, 7):
+        w = oracle.dense_weights(_abi.GX_DOPRI8, th)
+        assert w.shape == (14,) and np.all(w[1:5] == 0.0)
+        for k in range(1, 6):
+            assert abs(w @ c14 ** (k - 1) - th**k / k) < 1e-14, (th, k)
+
+
+def test_dopri5_dense_weights_match_scipy_rk45():
+    """diffrax's Dopri5 interpolant (quartic through y0, y1, both end slopes
+    and Shampine's midpoint) is the continuous extension scipy's RK45 uses:
+    its weights equal scipy's at every th."""
+    from scipy.integrate import RK45
+
+    for th in np.linspace(0.0, 1.0, 11):
+        ref = RK45.P @ np.array([th, th**2, th**3, th**4])
+        np.testing.assert_allclose(oracle.dense_weights(_abi.GX_DOPRI5, th), ref, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("method,hs,lo", [(Dopri8(), [8.0, 4.0, 2.0], 6.5), (Dopri5(), [2.0, 1.0, 0.5], 5.3)])
+def test_dense_output_converges_on_kepler(method, hs, lo):
+    """Off-grid saves against Kepler's exact solution: the error falls at a
+    high order as the fixed step halves (a wrong weight would cap it at 1-2)."""
+    pot = gp.KeplerPotential(m_tot=1e11)
+    GM = G * 1e11
+    y0 = np.array([8.0, 0.0, 0.5, 0.02, 0.18, 0.01])
+    errs = []
+    for h in hs:
+        ts = np.array([0.37 * h, 100 + 0.61 * h])
+        s = OrbitSolver(method, stepsize_controller=ConstantStepSize(), max_steps=None)
+        out, *_ = oracle.integrate(pot, s, y0[None], 0.0, 100 + h, ts=ts, dt0=h)
+        errs.append(max(np.max(np.abs(out[0, k] - _kepler_exact(GM, y0, ts[k]))) for k in range(2)))
+    orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(orders > lo), (errs, orders)
 
 
 def test_backward_integration_returns():
