@@ -31,7 +31,25 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 
-def cpu_baseline(pot, solver, w0, T, nsave, out_gpu, nsteps_gpu, seconds: float, threads: int):
+def host_cpu() -> dict:
+    """The host's CPU model (as lscpu reports it, from /proc/cpuinfo), the
+    cores this process may run on, and the cores a one-GPU job is allotted
+    (OMP_NUM_THREADS where the job sets it)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    visible = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return {"model": model, "cores_visible": visible, "cores_allotted": min(visible, share) if share > 0 else visible}
+
+
+def cpu_baseline(pot, solver, w0, T, nsave, out_gpu, nsteps_gpu, seconds: float, threads: int, seed: int = 12345):
     """The oracle on a bounded sample of the same workload (a seeded random
     subset of the orbits, integrated in chunks until `seconds` elapse); also
     compares those orbits with the GPU's output."""
@@ -39,7 +57,7 @@ def cpu_baseline(pot, solver, w0, T, nsave, out_gpu, nsteps_gpu, seconds: float,
 
     import oracle
 
-    rng = np.random.default_rng(12345)
+    rng = np.random.default_rng(seed)
     idx = rng.permutation(w0.shape[0])
     steps = 0
     done = 0
@@ -283,7 +301,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--n", type=int, default=1_048_576)
     ap.add_argument("--saves", type=int, default=64)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="oracle time on all allotted cores (a third of it again on one core)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the launch-policy search (galax_amd.tune) and use the fixed defaults")
+    ap.add_argument("--single-launch", type=int, default=3,
+                    help="after the timed region, passes timed as one launch on the whole device (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument(
         "--workload",
@@ -388,13 +411,38 @@ def main() -> int:
     # orbits: the persistent kernel's workgroups retire as its queue
     # empties, and the next launch's workgroups take their place.  Each
     # stream has its own output and workspace.
-    nfl = max(1, args.in_flight if args.in_flight is not None else (IN_FLIGHT if args.workload == "uneven" else 1))
-    parts = args.launch_parts if args.launch_parts is not None else min(nfl, 4)
-    _lib.check(_lib.lib().gx_set_launch_fraction(parts, None), "gx_set_launch_fraction")
+    ws_bytes = int(_lib.lib().gx_integrate_workspace_bytes(n))
+
+    def buffers(k):
+        return ([torch.empty((n, args.saves, 6), dtype=torch.float64, device="cuda") for _ in range(k)],
+                [torch.empty((ws_bytes + 7) // 8, dtype=torch.float64, device="cuda") for _ in range(k)])
+
+    tuning = None
+    explicit = args.in_flight is not None or args.launch_parts is not None or args.blocks_per_cu is not None
+    if args.workload == "uneven" and not explicit and not args.no_tune:
+        # the library's launch policy for this batch, searched on this device
+        # (cached per device, build and workload)
+        from galax_amd import tune
+
+        outs, wss = buffers(tune.MAX_IN_FLIGHT)
+
+        def launch(j):
+            gd.integrate_orbits(pot, w0d, t0d, Td, T=args.saves, solver=solver, order=od, validate_order=False,
+                                out=outs[j], workspace=wss[j])
+
+        tuning = tune.tune_batches(torch, launch, f"uneven:n={n}:saves={args.saves}:rtol=1e-8",
+                                   max_in_flight=tune.MAX_IN_FLIGHT,
+                                   log=lambda m: print(m, file=sys.stderr, flush=True))
+        nfl = int(tuning["settings"]["in_flight"])
+        parts = int(tuning["settings"]["parts"])
+        outs, wss = outs[:nfl], wss[:nfl]
+        torch.cuda.synchronize()
+    else:
+        nfl = max(1, args.in_flight if args.in_flight is not None else (IN_FLIGHT if args.workload == "uneven" else 1))
+        parts = args.launch_parts if args.launch_parts is not None else min(nfl, 4)
+        _lib.check(_lib.lib().gx_set_launch_fraction(parts, None), "gx_set_launch_fraction")
+        outs, wss = buffers(nfl)
     streams = [torch.cuda.Stream() for _ in range(nfl)]
-    outs = [torch.empty((n, args.saves, 6), dtype=torch.float64, device="cuda") for _ in range(nfl)]
-    wss = [torch.empty((int(_lib.lib().gx_integrate_workspace_bytes(n)) + 7) // 8, dtype=torch.float64,
-                       device="cuda") for _ in range(nfl)]
 
     def run(k, launch_events=None):
         # one pass on stream k % nfl: gx_integrate's queue reset, first-step
@@ -442,8 +490,8 @@ def main() -> int:
     naccept = res.naccept.cpu().numpy()
     total_steps = int(nsteps.sum())
     value = total_steps / (ms * 1e-3)
-    n_mini = n * max(args.saves - 2, 0)  # interior saves: one method step each
-    work = flops.f_step("dopri8") * total_steps + flops.f_save("dopri8") * n_mini
+    n_dense = n * max(args.saves - 2, 0)  # interior saves: the dense output
+    work = flops.f_step("dopri8") * total_steps + flops.f_dense("dopri8") * n_dense
     # the dominant kernel's own rate: its work per launch over its mean launch
     # duration (events on its stream; overlapping launches each count whole)
     launch_avg = float(np.mean(launch_ms))
@@ -478,10 +526,19 @@ def main() -> int:
             "blocks_per_cu": args.blocks_per_cu,
             "launch_ms_mean": launch_avg,
             "launch_ms": [round(x, 3) for x in launch_ms],
+            "tuning": None if tuning is None else {
+                "settings": tuning["settings"],
+                "ms_per_pass_in_search": tuning["ms_per_pass"],
+                "trials": len(tuning["trials"]),
+                "seconds": tuning["seconds"],
+                "cached": tuning["cached"],
+                "search": tuning["search"],
+                "fixed": tuning["fixed"],
+            },
         },
         "roofline": {
-            "bound": "mfma",
-            "pipe": "fp64 vector ALU (same 78.6 TFLOP/s peak as FP64 matrix)",
+            "bound": "fp64_valu",
+            "pipe": "FP64 vector ALU (no matrix instructions; MI355X FP64 vector peak 78.6 TFLOP/s)",
             "achieved": achieved,
             "peak": flops.PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
@@ -495,22 +552,58 @@ def main() -> int:
             "traffic_source": "profiles/k_integrate_mw_uneven_traffic.txt" if traffic else None,
             "achieved_basis": "FLOP per k_integrate launch / mean launch duration",
             "flop_per_step": flops.f_step("dopri8"),
-            "flop_per_save": flops.f_save("dopri8"),
+            "flop_per_interior_save": flops.f_dense("dopri8"),
+            "interior_saves": n_dense,
         },
     }
+    if args.workload == "uneven" and args.single_launch > 0:
+        # the configuration as stated: one persistent launch over the whole
+        # batch on the whole device, passes back to back
+        prev = __import__("ctypes").c_int32()
+        _lib.check(_lib.lib().gx_set_launch_fraction(1, __import__("ctypes").byref(prev)), "gx_set_launch_fraction")
+        run(0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(streams[0]):
+            e0.record()
+            for _ in range(args.single_launch):
+                r1 = gd.integrate_orbits(pot, w0d, t0d, Td, T=args.saves, solver=solver, order=od,
+                                         validate_order=False, out=outs[0], workspace=wss[0])
+            e1.record()
+        torch.cuda.synchronize()
+        ms1 = e0.elapsed_time(e1) / args.single_launch
+        steps1 = int(r1.nsteps.sum())
+        work1 = flops.f_step("dopri8") * steps1 + flops.f_dense("dopri8") * n_dense
+        line["single_launch"] = {
+            "in_flight": 1,
+            "launch_parts": 1,
+            "passes": args.single_launch,
+            "ms_per_pass": ms1,
+            "orbit_steps_per_s": steps1 / (ms1 * 1e-3),
+            "achieved_tflops": work1 / (ms1 * 1e-3) / 1e12,
+            "frac": work1 / (ms1 * 1e-3) / 1e12 / flops.PEAK_FP64_TFLOPS,
+        }
+        _lib.check(_lib.lib().gx_set_launch_fraction(parts, None), "gx_set_launch_fraction")
     if args.dump_steps:
         np.savez_compressed(args.dump_steps, nsteps=nsteps, order=order, T=T)
     if not args.no_cpu:
-        threads = min(16, len(os.sched_getaffinity(0)))
+        cpu = host_cpu()
+        threads = cpu["cores_allotted"]
         out_np = out.cpu().numpy()
         cps, done, el, parity = cpu_baseline(pot, solver, w0, T, args.saves, out_np, nsteps, args.cpu_seconds,
                                              threads)
+        cps1, done1, el1, _ = cpu_baseline(pot, solver, w0, T, args.saves, out_np, nsteps,
+                                           args.cpu_seconds / 3.0, 1, seed=54321)
         line["cpu_baseline"] = {
             "value": cps,
             "unit": "orbit-steps/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"{done} orbits drawn at random from the same workload, {el:.1f} s on {threads} threads",
+            "sample": f"{done} orbits drawn at random from the same workload, {el:.1f} s on {threads} threads "
+                      f"(every core allotted to this one-GPU job)",
+            "cpu_model": cpu["model"],
+            "cores_visible": cpu["cores_visible"],
+            "one_core": {"value": cps1, "sample": f"{done1} other orbits of the workload, {el1:.1f} s on 1 thread"},
         }
         line["parity_sample"] = parity
     print(json.dumps(line), flush=True)

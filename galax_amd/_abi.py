@@ -65,6 +65,8 @@ GX_TS_PER_ORBIT = 1
 GX_TS_LINSPACE = 2
 GX_TS_FINAL = 3
 
+GX_DENSE_RECORD = 50
+
 GX_MAX_COMPONENTS = 256
 GX_MAX_TRACKS = 8
 

@@ -1,10 +1,17 @@
-"""Build libgalaxhip.so (the gfx950 HIP library) in-tree with hipcc."""
+"""Build libgalaxhip.so (the gfx950 HIP library) in-tree with hipcc.
+
+Each HIP source compiles to its own object in parallel (every translation
+unit carries its own gfx950 code object), then the objects link into one
+shared library.
+"""
 
 from __future__ import annotations
 
 import os
 import shutil
 import subprocess
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -19,7 +26,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-Xclang",
     "-target-feature",
     "-Xclang",
@@ -35,17 +41,28 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: install ROCm or set HIPCC")
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile every HIP source into galax_amd/libgalaxhip.so."""
-    srcs = [str(CSRC / s) for s in SOURCES if (CSRC / s).exists()]
-    tmp = LIB_PATH.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", str(tmp), *srcs]
+def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stderr[-6000:]}")
-    os.replace(tmp, LIB_PATH)
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compile every HIP source into galax_amd/libgalaxhip.so."""
+    srcs = [CSRC / s for s in SOURCES if (CSRC / s).exists()]
+    cc = hipcc()
+    jobs = jobs or max(1, min(len(srcs), len(os.sched_getaffinity(0))))
+    with tempfile.TemporaryDirectory(prefix="galax_amd_build_") as tmpdir:
+        objs = [str(Path(tmpdir) / (s.stem + ".o")) for s in srcs]
+        cmds = [[cc, *HIPCC_FLAGS, "-c", str(s), "-o", o] for s, o in zip(srcs, objs)]
+        with ThreadPoolExecutor(max_workers=jobs) as ex:
+            for f in [ex.submit(_run, c, verbose) for c in cmds]:
+                f.result()
+        tmp = LIB_PATH.with_suffix(".so.tmp")
+        _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs], verbose)
+        os.replace(tmp, LIB_PATH)
     return LIB_PATH
 
 

@@ -42,12 +42,22 @@ SIGNATURES = {
         ctypes.c_int,
         [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     ),
+    "gx_integrate_dense": (
+        ctypes.c_int,
+        [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
+    ),
+    "gx_dense_eval": (ctypes.c_int, [_i32, _vp, _i64, _vp, _i64, _vp, _i32, _i32, _vp, _vp]),
+    "gx_integrate_shared_dense": (
+        ctypes.c_int,
+        [_vp, _vp, _vp, _f64, _f64, _vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
+    ),
     "gx_integrate_shared_workspace_bytes": (_i64, [_i64]),
     "gx_integrate_shared": (
         ctypes.c_int, [_vp, _vp, _vp, _f64, _f64, _vp, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     ),
     "gx_set_launch_shape": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "gx_set_launch_fraction": (ctypes.c_int, [_i32, ctypes.POINTER(_i32)]),
+    "gx_set_queue_policy": (ctypes.c_int, [_i64, _i32, ctypes.POINTER(_i64), ctypes.POINTER(_i32)]),
     "gx_device_info": (ctypes.c_int, [ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "gx_math_check": (ctypes.c_int, [_vp, _i64, _vp, _vp]),
     "gx_nbody_acceleration": (ctypes.c_int, [_vp, _vp, _vp, _f64, _f64, _i64, _vp, _vp]),
@@ -65,6 +75,10 @@ SIGNATURES = {
         ctypes.c_int,
         [_vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _i32, _i32, _i32, _vp, _vp, _i64, _vp, _i64,
          _vp, _i64, _vp],
+    ),
+    "gx_mlp_forward_fused": (
+        ctypes.c_int,
+        [_vp, _i32, _i32, _i64, _i32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i32, _vp, _i32, _vp],
     ),
     "gx_mlp_colsum": (ctypes.c_int, [_vp, _i64, _i64, _i64, _vp, _vp]),
     "gx_mlp_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _i64, _vp]),

@@ -21,12 +21,18 @@
 //     qb += Qb_k;  pb += h rs_k Qb_k
 //     gb_j += h^2 (A A)_kj Qb_k     (j < k)
 // which is the transpose of the forward recurrence in gx_integrate.hip.
-// Save times between steps are fed back through the adjoint of the step from
-// the last step start to the save time, as the forward pass produced them.
+// A save time inside a step is the step's dense output,
+//   q_s = q + th h p + h^2 th sum_l Wq_l(th) g_l,  p_s = p + h th sum_l Wp_l(th) g_l
+// over the stages and the end stage g_E = a(q1) (Dopri8), as the forward
+// pass produced it; its cotangents (cq, cp) add
+//   gb_l += h^2 th Wq_l cq + h th Wp_l cp,  qb += cq,  pb += th h cq + cp
+// and the end stage is reversed first: Q1b = -H(q1) gb_E, thb += ...,
+// qb += Q1b, pb += h (sum b) Q1b, gb_j += h^2 (bA)_j Q1b.
 #include <cmath>
 #include <utility>
 
 #include "gx_internal.hpp"
+#include "gx_rk.hpp"
 #include "gx_tableau.hpp"
 
 namespace gx {
@@ -53,7 +59,7 @@ struct AArgs {
   int ntheta;
   double* ckpt;  // [C][7][n]
   double* seg;   // [K][7][n]
-  double* stq;   // [S][3][n]
+  double* stq;   // [SF][3][n]: stage positions and the end position q1
   int* status;
 };
 
@@ -130,6 +136,11 @@ __device__ __forceinline__ void step_fwd(const DPot& P, const AArgs& A,
   const double q0 = fma(h, fma(sb, p[0], h * c0), q[0]);
   const double q1 = fma(h, fma(sb, p[1], h * c1), q[1]);
   const double q2 = fma(h, fma(sb, p[2], h * c2), q[2]);
+  if (REC && M::SF > S) {
+    A.stq[((long long)S * 3 + 0) * A.n + i] = q0;
+    A.stq[((long long)S * 3 + 1) * A.n + i] = q1;
+    A.stq[((long long)S * 3 + 2) * A.n + i] = q2;
+  }
   p[0] = fma(h, b0, p[0]);
   p[1] = fma(h, b1, p[1]);
   p[2] = fma(h, b2, p[2]);
@@ -148,30 +159,63 @@ __device__ __forceinline__ void adj_all(const DPot& P, const double (&Q)[3],
   (comp_adjoint<K>(P.comps[I], Q[0], Q[1], Q[2], w, Hw, th + 3 * I), ...);
 }
 
-// Adjoint of one step of size h from (q, p) given output cotangents
-// (qb, pb), updated in place to the cotangents of (q, p).
+// Adjoint of one step of size h from (q, p) given the step end's output
+// cotangents (qb, pb), updated in place to the cotangents of (q, p).  The
+// save times inside the step add gx (their stage cotangents over the SF
+// slots) and dq, dp (their direct cotangents of q and p); with no such save
+// (any_save false) those are zero and the end stage is skipped.
 template <class M, class E, int... K>
 __device__ __forceinline__ void step_adj(const DPot& P, const AArgs& A,
                                          long long i, double h,
                                          const double (&q)[3],
                                          const double (&p)[3], double (&qb)[3],
                                          double (&pb)[3],
-                                         double (&th)[kMaxTheta]) {
+                                         double (&th)[kMaxTheta],
+                                         const double (&gx)[M::SF][3],
+                                         const double (&dq)[3],
+                                         const double (&dp)[3],
+                                         bool any_save) {
   constexpr int S = M::S;
+  constexpr int SF = M::SF;
   {
     double qq[3] = {q[0], q[1], q[2]}, pp[3] = {p[0], p[1], p[2]};
     step_fwd<M, E, true>(P, A, i, h, qq, pp);
   }
-  double gb[S][3];
+  double gb[SF][3];
   const double h2 = h * h;
 #pragma unroll
-  for (int j = 0; j < S; ++j)
+  for (int j = 0; j < SF; ++j)
 #pragma unroll
     for (int m = 0; m < 3; ++m)
-      gb[j][m] = fma(h * M::k.b[j], pb[m], h2 * M::k.ba[j] * qb[m]);
+      gb[j][m] = (j < S ? fma(h * M::k.b[j], pb[m], h2 * M::k.ba[j] * qb[m]) : 0.0) + gx[j][m];
   constexpr double sb = M::k.sb;
 #pragma unroll
-  for (int m = 0; m < 3; ++m) pb[m] = fma(h * sb, qb[m], pb[m]);
+  for (int m = 0; m < 3; ++m) {
+    pb[m] = fma(h * sb, qb[m], pb[m]) + dp[m];
+    qb[m] += dq[m];
+  }
+  if constexpr (SF > S) {
+    if (any_save) {
+      // the end stage g_E = a(q1), q1 = q + h (sb p + h sum (bA)_j g_j)
+      double Q[3], Hw[3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) Q[m] = A.stq[((long long)S * 3 + m) * A.n + i];
+      adj_all<K...>(P, Q, gb[S], Hw, th, std::make_index_sequence<sizeof...(K)>{});
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        qb[m] -= Hw[m];
+        pb[m] = fma(-h * sb, Hw[m], pb[m]);
+      }
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        const double a = M::k.ba[j];
+        if (a != 0.0) {
+#pragma unroll
+          for (int m = 0; m < 3; ++m) gb[j][m] = fma(-h2 * a, Hw[m], gb[j][m]);
+        }
+      }
+    }
+  }
 #pragma unroll
   for (int k = S - 1; k >= 0; --k) {
     double Q[3], Hw[3];
@@ -299,20 +343,34 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(DPot P, AArgs A) {
           }
           --si;
         }
-        step_adj<M, E, K...>(P, A, i, tau_np1 - tau_n, qn, pn, qb, pb, th);
-        // interior saves (a step from the step start to the save time)
+        // interior saves: the step's dense output at th = (t_s - tau_n) / h
+        const double h = tau_np1 - tau_n;
+        double gx[M::SF][3], dq[3] = {0.0, 0.0, 0.0}, dp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int l = 0; l < M::SF; ++l) gx[l][0] = gx[l][1] = gx[l][2] = 0.0;
+        bool any_save = false;
         while (si >= 0) {
           const double tsv = dir * save_time_a(A, i, si, t0v, t1v);
           if (!(tsv > tau_n && tsv < tau_np1)) break;
           double cq[3], cp[3];
           cot(si, cq, cp);
-          step_adj<M, E, K...>(P, A, i, tsv - tau_n, qn, pn, cq, cp, th);
+          const double tht = (tsv - tau_n) / h;
+          double wq[M::SF], wp[M::SF];
+          dense_weights<M>(tht, wq, wp);
+          const double ht = h * tht, hht = h * ht;
+#pragma unroll
+          for (int l = 0; l < M::SF; ++l)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+              gx[l][k] = fma(hht * wq[l], cq[k], fma(ht * wp[l], cp[k], gx[l][k]));
           for (int k = 0; k < 3; ++k) {
-            qb[k] += cq[k];
-            pb[k] += cp[k];
+            dq[k] += cq[k];
+            dp[k] = fma(ht, cq[k], dp[k] + cp[k]);
           }
+          any_save = true;
           --si;
         }
+        step_adj<M, E, K...>(P, A, i, h, qn, pn, qb, pb, th, gx, dq, dp, any_save);
       }
     }
     // saves at (or before) the start
@@ -380,7 +438,7 @@ extern "C" {
 int64_t gx_adjoint_workspace_bytes(int64_t n, int64_t max_steps, int32_t K) {
   if (n < 0 || max_steps < 0 || K < 1) return 0;
   const int64_t C = max_steps / K + 2;
-  return 8 * n * (7 * C + 7 * (int64_t)K + 3 * 13) + 256;
+  return 8 * n * (7 * C + 7 * (int64_t)K + 3 * 14) + 256;
 }
 
 int gx_adjoint(const gx_pot* pot, const gx_solver_cfg* cfg, const double* w0,

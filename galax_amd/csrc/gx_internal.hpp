@@ -40,6 +40,8 @@ struct LaunchShape {
   int block = 256;
   int blocks_per_cu = 4;
   int parts = 1;  // a persistent launch takes 1/parts of the device
+  long long head_lanes = 4096;  // lanes that start on the queue head
+  int chunk = 1;  // tickets a lane takes from the work queue at a time
 };
 LaunchShape& launch_shape();
 

@@ -655,6 +655,18 @@ int gx_set_launch_fraction(int32_t parts, int32_t* prev_parts) {
   return GX_OK;
 }
 
+int gx_set_queue_policy(int64_t head_lanes, int32_t chunk,
+                        int64_t* prev_head_lanes, int32_t* prev_chunk) {
+  LaunchShape& s = launch_shape();
+  if (prev_head_lanes) *prev_head_lanes = s.head_lanes;
+  if (prev_chunk) *prev_chunk = s.chunk;
+  if (head_lanes > ((int64_t)1 << 30) || chunk < 0 || chunk > 64)
+    return GX_EINVAL;
+  if (head_lanes >= 0) s.head_lanes = head_lanes;
+  if (chunk > 0) s.chunk = chunk;
+  return GX_OK;
+}
+
 int gx_math_check(const double* x, int64_t n, double* out, void* stream) {
   if (!x || !out || n < 0) return GX_EINVAL;
   if (n == 0) return GX_OK;

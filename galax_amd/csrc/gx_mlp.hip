@@ -191,6 +191,116 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
       }
 }
 
+// The whole inference pass of an MLP of hidden width 256 in one kernel: a
+// block takes 64 points and keeps their activations in LDS (two 64 x 256
+// BF16 buffers, used in turn), so that activations never go to HBM; every
+// hidden-to-hidden layer runs on the matrix cores with the same operand
+// layout as k_gemm, the layer's BF16 weights read from L2 (each layer's
+// 128 KB is shared by every block).  Each wave owns 64 output columns of a
+// layer: 4 x 4 MFMA tiles.  The input layer (8 padded inputs) and the output
+// layer (a few columns) are short dot products in FP32.  Rounding as the
+// layer-by-layer path: FP32 accumulation, bias + SiLU in FP32, BF16
+// activations.
+constexpr int FH = 256;      // hidden width
+constexpr int FM = 64;       // points per block
+constexpr int FLD = FH + 8;  // padded LDS row (bf16 elements)
+constexpr int kMaxFusedLayers = 8;
+
+struct FusedArgs {
+  const __bf16* X;  // [n][ldx] inputs (BF16)
+  int ldx, nin;     // row stride, inputs used
+  long long n;
+  int depth;        // hidden layers
+  int nout;
+  const __bf16* W[kMaxFusedLayers + 1];  // layer l: [out][in] row-major
+  const float* b[kMaxFusedLayers + 1];
+  float* Y;         // [n][ldy] FP32 outputs
+  int ldy;
+};
+
+__global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs f) {
+  __shared__ __bf16 act[2][FM][FLD];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const long long m0 = (long long)blockIdx.x * FM;
+  // ---- input layer: thread tid owns hidden unit tid for all 64 points
+  {
+    const int c = tid;
+    float w[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      w[k] = k < f.nin ? (float)f.W[0][c * f.nin + k] : 0.0f;
+    const float bias = f.b[0][c];
+    for (int r = 0; r < FM; ++r) {
+      const long long m = m0 + r;
+      float z = 0.0f;
+      if (m < f.n) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (k < f.nin) z = fmaf((float)f.X[m * f.ldx + k], w[k], z);
+      }
+      act[0][r][c] = (__bf16)silu(z + bias);
+    }
+  }
+  int cur = 0;
+  // ---- hidden-to-hidden layers on the matrix cores
+  for (int l = 1; l < f.depth; ++l) {
+    __syncthreads();
+    const __bf16* W = f.W[l];
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int k0 = 0; k0 < FH; k0 += 32) {
+      const int kk = k0 + (lane >> 4) * 8;
+      bf16x8 af[4], bw[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int n = wave * 64 + b * 16 + (lane & 15);
+        bw[b] = *reinterpret_cast<const bf16x8*>(W + n * FH + kk);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        af[a] = *reinterpret_cast<const bf16x8*>(&act[cur][a * 16 + (lane & 15)][kk]);
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bw[b], acc[a][b], 0, 0, 0);
+    }
+    const float* bias = f.b[l];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int n = wave * 64 + b * 16 + (lane & 15);
+      const float bn = bias[n];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          act[cur ^ 1][a * 16 + (lane >> 4) * 4 + j][n] = (__bf16)silu(acc[a][b][j] + bn);
+    }
+    cur ^= 1;
+  }
+  __syncthreads();
+  // ---- output layer: one (point, output) pair per thread
+  const __bf16* Wo = f.W[f.depth];
+  for (int t = tid; t < FM * f.nout; t += 256) {
+    const int r = t / f.nout, o = t - r * f.nout;
+    const long long m = m0 + r;
+    if (m >= f.n) continue;
+    float z = 0.0f;
+    for (int k = 0; k < FH; k += 8) {
+      const bf16x8 hv = *reinterpret_cast<const bf16x8*>(&act[cur][r][k]);
+      const bf16x8 wv = *reinterpret_cast<const bf16x8*>(Wo + o * FH + k);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) z = fmaf((float)hv[j], (float)wv[j], z);
+    }
+    f.Y[m * f.ldy + o] = z + f.b[f.depth][o];
+  }
+}
+
 // db[n] += sum_m dZ[m, n] (bf16 in, fp32 out) -- partial column sums per
 // block combined with atomics.
 __global__ void k_colsum(const __bf16* __restrict__ dz, long long ld, long long M,
@@ -445,6 +555,38 @@ int gx_mlp_gemm(const void* A, int64_t sam, int64_t sak, const void* B,
   if (mt > 0x7fffffffLL || nt > 65535) return GX_EINVAL;
   dim3 grid((unsigned)mt, (unsigned)nt, (unsigned)ksplit);
   hipLaunchKernelGGL(k_gemm<true>, grid, dim3(256), 0, (hipStream_t)stream, g);
+  return check(hipGetLastError());
+}
+
+int gx_mlp_forward_fused(const void* X, int32_t ldx, int32_t nin, int64_t n,
+                         int32_t depth, const void* const* W,
+                         const float* const* b, int32_t nout, float* Y,
+                         int32_t ldy, void* stream) {
+  if (n < 0 || depth < 1 || depth > kMaxFusedLayers || nin < 1 || nin > 8 ||
+      ldx < nin || nout < 1 || nout > 16 || ldy < nout)
+    return GX_EINVAL;
+  if (n == 0) return GX_OK;
+  if (!X || !W || !b || !Y) return GX_EINVAL;
+  FusedArgs f{};
+  f.X = static_cast<const __bf16*>(X);
+  f.ldx = ldx;
+  f.nin = nin;
+  f.n = n;
+  f.depth = depth;
+  f.nout = nout;
+  f.Y = Y;
+  f.ldy = ldy;
+  for (int l = 0; l <= depth; ++l) {
+    if (!W[l] || !b[l]) return GX_EINVAL;
+    // 16-byte fragment loads of the hidden and output layers' rows
+    if (l > 0 && (reinterpret_cast<uintptr_t>(W[l]) & 15)) return GX_EINVAL;
+    f.W[l] = static_cast<const __bf16*>(W[l]);
+    f.b[l] = b[l];
+  }
+  const long long blocks = (n + FM - 1) / FM;
+  if (blocks > 0x7fffffffLL) return GX_EINVAL;
+  hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)blocks), dim3(256), 0,
+                     (hipStream_t)stream, f);
   return check(hipGetLastError());
 }
 

@@ -19,6 +19,7 @@
 
 #include "gx_internal.hpp"
 #include "gx_potential.hpp"
+#include "gx_rk.hpp"
 #include "gx_tableau.hpp"
 
 namespace gx {
@@ -47,8 +48,7 @@ struct NArgs {
   double* p;
   double* q1;        // end of the current step [n][3]
   double* p1;
-  double* G;         // stage accelerations [S][n][3]
-  double* keep;      // the step's end state while a save's method step runs [n][6]
+  double* G;         // stage accelerations [SF][n][3] (slot SF-1: the end stage)
 };
 
 struct NbShared {
@@ -183,8 +183,9 @@ __global__ void __launch_bounds__(kNbThreads) k_nbody(DPot P, NArgs A) {
   int status = GX_SUCCESS;
   long long nst = 0, nacc = 0;
   if (A.t0 != A.t1) {
-    // first step: dt0, or diffrax's initial-step heuristic over the system
-    double h0 = A.dt0;
+    // first step: dt0; dtmin if the controller sets one (as the orbit
+    // integrators); else diffrax's initial-step heuristic over the system
+    double h0 = A.dt0 > 0.0 ? A.dt0 : (PID && A.dtmin > 0.0 ? A.dtmin : 0.0);
     if (!(h0 > 0.0)) {
       double s0 = 0.0, s1 = 0.0;
       for (long long j = tid; j < n; j += kNbThreads) {
@@ -293,29 +294,47 @@ __global__ void __launch_bounds__(kNbThreads) k_nbody(DPot P, NArgs A) {
         next_tau1 = tnext + A.dt0;
       }
       if (keep) {
-        // saves in (tau, tnext]: the step end, or a method step from tau
+        // saves in (tau, tnext]: the step end, or the step's dense output
+        // (diffrax's interpolant), which reads the end stage a(q1, tnext)
+        bool have_end = M::SF == M::S;  // Dopri5's last stage is at q1
         while (si < A.T && dir * A.ts[si] <= tnext) {
           const double ts_tau = dir * A.ts[si];
           if (ts_tau == tnext) {
             nb_write(A, si, dir, A.q1, A.p1);
           } else {
-            // the save's method step overwrites q1/p1: keep the step's end
-            // state aside meanwhile
-            for (long long j = tid; j < n; j += kNbThreads)
-              for (int m = 0; m < 3; ++m) {
-                A.keep[6 * j + m] = A.q1[3 * j + m];
-                A.keep[6 * j + 3 + m] = A.p1[3 * j + m];
+            if (!have_end) {
+              for (long long j = tid; j < n; j += kNbThreads) {
+                S.x[j] = A.q1[3 * j];
+                S.y[j] = A.q1[3 * j + 1];
+                S.z[j] = A.q1[3 * j + 2];
               }
-            __syncthreads();
-            nb_eval<M, false>(P, S, A, dir, tau, ts_tau - tau);
-            __syncthreads();
-            nb_write(A, si, dir, A.q1, A.p1);
-            __syncthreads();
-            for (long long j = tid; j < n; j += kNbThreads)
-              for (int m = 0; m < 3; ++m) {
-                A.q1[3 * j + m] = A.keep[6 * j + m];
-                A.p1[3 * j + m] = A.keep[6 * j + 3 + m];
+              __syncthreads();
+              for (long long j = tid; j < n; j += kNbThreads) {
+                double* g = A.G + ((long long)(M::SF - 1) * n + j) * 3;
+                nb_accel(P, S, n, A.eps2, j, S.x[j], S.y[j], S.z[j], dir * tnext, g[0], g[1], g[2]);
               }
+              __syncthreads();
+              have_end = true;
+            }
+            double wq[M::SF], wp[M::SF];
+            const double th = (ts_tau - tau) / h;
+            dense_weights<M>(th, wq, wp);
+            const double ht = h * th, hht = h * ht;
+            for (long long j = tid; j < n; j += kNbThreads) {
+              double sq_[3] = {0.0, 0.0, 0.0}, sp_[3] = {0.0, 0.0, 0.0};
+              for (int l = 0; l < M::SF; ++l)
+                for (int m = 0; m < 3; ++m) {
+                  const double gl = A.G[((long long)l * n + j) * 3 + m];
+                  sq_[m] = fma(wq[l], gl, sq_[m]);
+                  sp_[m] = fma(wp[l], gl, sp_[m]);
+                }
+              double* o = A.out + (j * A.T + si) * 6;
+              for (int m = 0; m < 3; ++m) {
+                const double q0 = A.q[3 * j + m], p0 = A.p[3 * j + m];
+                o[m] = fma(hht, sq_[m], fma(ht, p0, q0));
+                o[3 + m] = dir * fma(ht, sp_[m], p0);
+              }
+            }
           }
           ++si;
         }
@@ -394,7 +413,7 @@ using namespace gx;
 extern "C" {
 
 int64_t gx_nbody_workspace_bytes(int64_t n) {
-  return n < 0 ? 0 : 256 + 8 * n * (12 + 3 * 13 + 6);
+  return n < 0 ? 0 : 256 + 8 * n * (12 + 3 * 14);
 }
 
 int gx_nbody_acceleration(const gx_pot* ext, const double* xyz, const double* gm,
@@ -454,7 +473,6 @@ int gx_nbody_integrate(const gx_pot* ext, const double* gm, double eps,
   A.q1 = w + 6 * n;
   A.p1 = w + 9 * n;
   A.G = w + 12 * n;
-  A.keep = w + 51 * n;
   hipStream_t s = (hipStream_t)stream;
   const bool pid = cfg->controller == GX_PID;
   if (cfg->method == GX_DOPRI8) {

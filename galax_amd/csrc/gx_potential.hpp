@@ -743,7 +743,9 @@ struct EvalTable {
   static constexpr bool kLarge = FULL;
   static constexpr bool kFrames = FULL;
   static constexpr bool kInlineFirstStep = false;
-  static constexpr int kWavesPerSimd = 0;  // occupancy target (0: none)
+  // occupancy target (0: none): the frame-less table kernels keep two waves
+  // per SIMD (without the target the compiler takes one wave and the AGPRs)
+  static constexpr int kWavesPerSimd = FULL ? 0 : 2;
   __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
                                              double z, double t, double& ax,
                                              double& ay, double& az) {

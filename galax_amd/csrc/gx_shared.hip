@@ -15,11 +15,13 @@
 // workgroup to finish (a device-scope ticket) reduces the partials in a fixed
 // order and runs the controller for the whole batch, writing a small control
 // record that the next launch reads.  No lane ever waits on another
-// workgroup, so a launch always drains.  The state is double-buffered: an
-// attempted step writes its candidate to the other buffer, and acceptance
-// swaps the buffers, which keeps the previous state for the save times that
-// fall inside the accepted step (filled by the next launch with a method
-// step from the accepted step's start, as the independent integrator does).
+// workgroup, so a launch always drains.  The state (q, p and the step
+// start's acceleration) is double-buffered: an attempted step writes its
+// end state and end stage to the other buffer, and acceptance swaps the
+// buffers.  Save times inside an attempted step are written from its dense
+// output before the decision is known: a rejected attempt's values are
+// overwritten by the accepted step that later covers those times, and an
+// early stop marks every save past the last accepted step NaN.
 // The host launches steps in batches and reads the record's `done` flag
 // between batches; launches after the end return at once.
 #include <cmath>
@@ -37,14 +39,13 @@ constexpr long long kShStepCap = 10000000;
 // The control record (device memory, first 256 bytes of the workspace).
 struct SCtl {
   double tau, tnext, tau1;  // solver time, end of the next attempt, end time
-  double tau_acc0;          // start of the last accepted step
   double h0, d1;            // automatic first step: trial step, |f| norm
   double prev_inv, prev_prev_inv;
   long long nsteps, naccept;
   int status;
-  int si, si_end;  // save times [si, si_end) fall in the last accepted step
-  int cur;         // buffer that holds the current state
-  int at_dtmin, finished, done, pad;
+  int si;   // save times before si are final (at or before tau)
+  int cur;  // buffer that holds the current state
+  int at_dtmin, finished, done;
 };
 static_assert(sizeof(SCtl) <= 256, "control record");
 
@@ -65,9 +66,14 @@ struct SArgs {
   SCtl* ctl;
   unsigned int* counter;
   double* part;  // two partial sums per workgroup
-  double* y;     // two state buffers of n x 6 (solver variables)
-  double* f0;    // n x 3 accelerations at the start (automatic first step)
+  double* y;     // two state buffers of n x 9: q, p (solver variables), and
+                 // a(q, tau), the step start's first stage
+  double* rec;   // dense records [n][rec_cap][kDenseRec], or null
+  long long rec_cap;
+  long long* nrec;
 };
+
+constexpr int kRow = 9;  // doubles per orbit in a state buffer
 
 __device__ __forceinline__ void write_save(const SArgs& A, long long i, int s,
                                            const double (&q)[3],
@@ -82,14 +88,28 @@ __device__ __forceinline__ void write_save(const SArgs& A, long long i, int s,
 }
 
 __device__ __forceinline__ void load_state(const double* y, long long i,
-                                           double (&q)[3], double (&p)[3]) {
-  const double* r = y + 6 * i;
-  q[0] = r[0];
-  q[1] = r[1];
-  q[2] = r[2];
-  p[0] = r[3];
-  p[1] = r[4];
-  p[2] = r[5];
+                                           double (&q)[3], double (&p)[3],
+                                           double (&a)[3]) {
+  const double* r = y + kRow * i;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    q[m] = r[m];
+    p[m] = r[m + 3];
+    a[m] = r[m + 6];
+  }
+}
+
+__device__ __forceinline__ void store_state(double* y, long long i,
+                                            const double (&q)[3],
+                                            const double (&p)[3],
+                                            const double (&a)[3]) {
+  double* r = y + kRow * i;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    r[m] = q[m];
+    r[m + 3] = p[m];
+    r[m + 6] = a[m];
+  }
 }
 
 // Sum of v over the workgroup, in a fixed order (every thread gets it).
@@ -136,9 +156,14 @@ __device__ bool last_block_sums(const SArgs& A, double a, double b,
   return true;
 }
 
-// Copy the initial states in (velocities times dir), write the saves at the
-// start time, initialise the control record.
-__global__ void __launch_bounds__(kShBlock) k_sh_setup(SArgs A) {
+// Copy the initial states in (velocities times dir) with their start
+// accelerations (the first step's first stage, by the step's own
+// evaluator), write the saves at the start time, initialise the control
+// record.  The first step is dt0 if given, else dtmin if the controller sets
+// one (as the independent integrator), else the automatic rule's launches
+// set it.
+template <class E>
+__global__ void __launch_bounds__(kShBlock) k_sh_setup(DPot P, SArgs A) {
   const long long i = (long long)blockIdx.x * kShBlock + threadIdx.x;
   int si = 0;
   while (si < A.T && A.dir * A.ts[si] <= A.tau0) ++si;
@@ -147,11 +172,9 @@ __global__ void __launch_bounds__(kShBlock) k_sh_setup(SArgs A) {
     const double* w = A.w0 + 6 * i;
     const double q[3] = {w[0], w[1], w[2]};
     const double p[3] = {A.dir * w[3], A.dir * w[4], A.dir * w[5]};
-    double* y = A.y + 6 * i;
-    for (int m = 0; m < 3; ++m) {
-      y[m] = q[m];
-      y[m + 3] = p[m];
-    }
+    double a[3] = {0.0, 0.0, 0.0};
+    if (!same) E::acc(P, q[0], q[1], q[2], A.dir * A.tau0, a[0], a[1], a[2]);
+    store_state(A.y, i, q, p, a);
     const int ns = same ? A.T : si;
     for (int s = 0; s < ns; ++s) write_save(A, i, s, q, p);
   }
@@ -159,15 +182,14 @@ __global__ void __launch_bounds__(kShBlock) k_sh_setup(SArgs A) {
     SCtl c{};
     c.tau = A.tau0;
     c.tau1 = A.tau1;
-    c.tau_acc0 = A.tau0;
     c.prev_inv = 1.0;
     c.prev_prev_inv = 1.0;
     c.status = GX_SUCCESS;
-    c.si = c.si_end = same ? A.T : si;
+    c.si = same ? A.T : si;
     c.finished = same ? 1 : 0;
     c.done = same ? 1 : 0;
-    if (A.dt0 > 0.0) {  // else the automatic first step sets tnext
-      double h = A.dt0;
+    if (A.dt0 > 0.0 || A.dtmin > 0.0) {
+      double h = A.dt0 > 0.0 ? A.dt0 : A.dtmin;
       h = fmin(h, A.dtmax);
       if (A.dtmin > 0.0) {
         c.at_dtmin = h <= A.dtmin ? 1 : 0;
@@ -183,16 +205,13 @@ __global__ void __launch_bounds__(kShBlock) k_sh_setup(SArgs A) {
 // diffrax's automatic first step over the whole batch (Hairer, Norsett &
 // Wanner II.4), in two launches: the norms of y and f(y) and a trial step,
 // then the norm of the change of f over the trial step.
-template <bool FULL>
-__global__ void __launch_bounds__(kShBlock) k_sh_first_a(DPot P, SArgs A) {
+__global__ void __launch_bounds__(kShBlock) k_sh_first_a(SArgs A) {
   const long long i = (long long)blockIdx.x * kShBlock + threadIdx.x;
   double s0 = 0.0, s1 = 0.0;
   if (i < A.n) {
     double q[3], p[3], a[3];
-    load_state(A.y, i, q, p);
-    accel<FULL>(P, q[0], q[1], q[2], A.dir * A.tau0, a[0], a[1], a[2]);
+    load_state(A.y, i, q, p, a);
     for (int m = 0; m < 3; ++m) {
-      A.f0[3 * i + m] = a[m];
       const double sq = A.atol + fabs(q[m]) * A.rtol;
       const double sp = A.atol + fabs(p[m]) * A.rtol;
       s0 += (q[m] / sq) * (q[m] / sq) + (p[m] / sp) * (p[m] / sp);
@@ -211,7 +230,7 @@ __global__ void __launch_bounds__(kShBlock) k_sh_first_a(DPot P, SArgs A) {
   }
 }
 
-template <bool FULL>
+template <class E>
 __global__ void __launch_bounds__(kShBlock) k_sh_first_b(DPot P, SArgs A,
                                                          double order) {
   const long long i = (long long)blockIdx.x * kShBlock + threadIdx.x;
@@ -219,14 +238,13 @@ __global__ void __launch_bounds__(kShBlock) k_sh_first_b(DPot P, SArgs A,
   double s2 = 0.0;
   if (i < A.n) {
     double q[3], p[3], a0[3], a1[3], q1[3], p1[3];
-    load_state(A.y, i, q, p);
+    load_state(A.y, i, q, p, a0);
     for (int m = 0; m < 3; ++m) {
-      a0[m] = A.f0[3 * i + m];
       q1[m] = q[m] + p[m] * h0;
       p1[m] = p[m] + a0[m] * h0;
     }
-    accel<FULL>(P, q1[0], q1[1], q1[2], A.dir * (A.tau0 + h0), a1[0], a1[1],
-                a1[2]);
+    E::acc(P, q1[0], q1[1], q1[2], A.dir * (A.tau0 + h0), a1[0], a1[1],
+           a1[2]);
     for (int m = 0; m < 3; ++m) {
       const double sq = A.atol + fabs(q[m]) * A.rtol;
       const double sp = A.atol + fabs(p[m]) * A.rtol;
@@ -260,8 +278,8 @@ __device__ __forceinline__ double clip_end(double tprev, double tnext,
                               : tnext;
 }
 
-// One attempted step of the whole batch (plus the save times of the step
-// accepted before it).
+// One attempted step of the whole batch, with the save times inside it
+// written from the step's dense output (before the decision; see above).
 template <class M, bool PID, class E>
 __global__ void __launch_bounds__(kShBlock) k_sh_step(DPot P, SArgs A) {
   const SCtl c = *A.ctl;
@@ -269,51 +287,37 @@ __global__ void __launch_bounds__(kShBlock) k_sh_step(DPot P, SArgs A) {
   const long long i = (long long)blockIdx.x * kShBlock + threadIdx.x;
   const bool valid = i < A.n;
   const double dir = A.dir;
-  const double* ycur = A.y + (long long)c.cur * 6 * A.n;
-  double* yoth = A.y + (long long)(c.cur ^ 1) * 6 * A.n;
-  const int nsave = c.si_end - c.si;
-  const int nev = nsave + (c.finished ? 0 : 1);
+  const double* ycur = A.y + (long long)c.cur * kRow * A.n;
+  double* yoth = A.y + (long long)(c.cur ^ 1) * kRow * A.n;
   double sq = 0.0, bad = 0.0;
-  // one call site for the method: the saves, then the step (uniform trips)
-  for (int e = 0; e < nev; ++e) {
-    const bool is_save = e < nsave;
-    const int s = c.si + e;
-    double tb, h;
-    const double* base;
-    if (is_save) {
-      const double ts_tau = dir * A.ts[s];
-      if (ts_tau == c.tau) {  // the accepted step's end
-        if (valid) {
-          double q[3], p[3];
-          load_state(ycur, i, q, p);
-          write_save(A, i, s, q, p);
-        }
-        continue;
-      }
-      tb = c.tau_acc0;
-      h = ts_tau - tb;
-      base = yoth;
-    } else {
-      tb = c.tau;
-      h = c.tnext - c.tau;
-      base = ycur;
-    }
-    if (!valid) continue;
-    double q[3], p[3], q1[3], p1[3], eq[3], ep[3];
-    load_state(base, i, q, p);
-    rk_eval<M, E, PID>(P, dir, tb, h, q, p, q1, p1, eq, ep);
-    if (is_save) {
-      write_save(A, i, s, q1, p1);
-      continue;
-    }
-    double* y1 = yoth + 6 * i;
+  if (valid && !c.finished) {
+    constexpr int SF = M::SF;
+    const double h = c.tnext - c.tau;
+    double q[3], p[3], g[SF][3], q1[3], p1[3], eq[3], ep[3];
+    load_state(ycur, i, q, p, g[0]);
+    rk_step<M, E, PID>(P, dir, c.tau, h, c.tnext, q, p, g, q1, p1, eq, ep);
+    rk_end_stage<M, E>(P, dir, c.tnext, q1, g);
+    store_state(yoth, i, q1, p1, g[SF - 1]);
+    // this attempt's record goes to the next free slot; a rejection leaves
+    // the slot to the next attempt
+    if (A.rec && c.naccept < A.rec_cap)
+      write_record<M>(A.rec + (i * A.rec_cap + c.naccept) * kDenseRec, dir,
+                      c.tau, h, q, p, g);
     bool finite = true;
-    for (int m = 0; m < 3; ++m) {
-      y1[m] = q1[m];
-      y1[m + 3] = p1[m];
+    for (int m = 0; m < 3; ++m)
       finite = finite && isfinite(q1[m]) && isfinite(p1[m]);
-    }
     bad = finite ? 0.0 : 1.0;
+    for (int s = c.si; s < A.T; ++s) {
+      const double ts_tau = dir * A.ts[s];
+      if (ts_tau > c.tnext) break;
+      if (ts_tau == c.tnext) {
+        write_save(A, i, s, q1, p1);
+      } else {
+        double qs[3], ps[3];
+        dense_eval<M>(div(ts_tau - c.tau, h), h, q, p, g, qs, ps);
+        write_save(A, i, s, qs, ps);
+      }
+    }
     if (PID) {
       for (int m = 0; m < 3; ++m) {
         const double uq = eq[m] / (A.atol + fmax(fabs(q[m]), fabs(q1[m])) * A.rtol);
@@ -326,7 +330,6 @@ __global__ void __launch_bounds__(kShBlock) k_sh_step(DPot P, SArgs A) {
   if (!last_block_sums(A, sq, bad, SQ, NB)) return;
   if (threadIdx.x != 0) return;
   SCtl n = c;
-  n.si = c.si_end;
   if (c.finished) {
     n.done = 1;
   } else {
@@ -365,11 +368,14 @@ __global__ void __launch_bounds__(kShBlock) k_sh_step(DPot P, SArgs A) {
       }
       if (keep) {
         n.cur = c.cur ^ 1;
-        n.tau_acc0 = c.tau;
         n.naccept = c.naccept + 1;
-        int s = c.si_end;
+        if (A.rec && n.naccept > A.rec_cap) {
+          n.status = GX_MAX_STEPS_REACHED;  // out of record slots
+          n.finished = 1;
+        }
+        int s = c.si;
         while (s < A.T && dir * A.ts[s] <= c.tnext) ++s;
-        n.si_end = s;
+        n.si = s;
       }
       const double tprev = fmin(keep ? c.tnext : c.tau, c.tau1);
       n.tnext = clip_end(tprev, next_t1, c.tau1, keep);
@@ -401,6 +407,7 @@ __global__ void __launch_bounds__(kShBlock) k_sh_finish(SArgs A) {
   if (A.status) A.status[i] = c.status;
   if (A.nsteps) A.nsteps[i] = c.nsteps;
   if (A.naccept) A.naccept[i] = c.naccept;
+  if (A.nrec) A.nrec[i] = c.naccept < A.rec_cap ? c.naccept : A.rec_cap;
 }
 
 // Two ints of pinned host memory per host thread for the done flags
@@ -416,31 +423,57 @@ int* pinned_flags() {
   return flags;
 }
 
+// A stream of this host thread used only to capture the step graph (the
+// graph is then launched on the caller's stream); default priority.
+hipStream_t capture_stream() {
+  thread_local hipStream_t s = nullptr;
+  if (!s && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+    s = nullptr;
+  return s;
+}
+
+constexpr int kGraphSteps = 32;
+
 template <class M, bool PID, class E>
 int run(const gx_pot* pot, SArgs& A, bool first_step, hipStream_t stream) {
   const unsigned blocks = (unsigned)((A.n + kShBlock - 1) / kShBlock);
-  hipLaunchKernelGGL(k_sh_setup, dim3(blocks), dim3(kShBlock), 0, stream, A);
+  hipLaunchKernelGGL(k_sh_setup<E>, dim3(blocks), dim3(kShBlock), 0, stream,
+                     pot->dev, A);
   if (first_step) {
-    if (pot->basic) {
-      hipLaunchKernelGGL(k_sh_first_a<false>, dim3(blocks), dim3(kShBlock), 0,
-                         stream, pot->dev, A);
-      hipLaunchKernelGGL(k_sh_first_b<false>, dim3(blocks), dim3(kShBlock), 0,
-                         stream, pot->dev, A, (double)M::order);
-    } else {
-      hipLaunchKernelGGL(k_sh_first_a<true>, dim3(blocks), dim3(kShBlock), 0,
-                         stream, pot->dev, A);
-      hipLaunchKernelGGL(k_sh_first_b<true>, dim3(blocks), dim3(kShBlock), 0,
-                         stream, pot->dev, A, (double)M::order);
-    }
+    hipLaunchKernelGGL(k_sh_first_a, dim3(blocks), dim3(kShBlock), 0, stream,
+                       A);
+    hipLaunchKernelGGL(k_sh_first_b<E>, dim3(blocks), dim3(kShBlock), 0,
+                       stream, pot->dev, A, (double)M::order);
   }
   if (hipGetLastError() != hipSuccess) return GX_ELAUNCH;
   // Step launches go out in batches; after each batch the record's `done`
   // flag is copied to pinned host memory behind an event.  Two batches are
   // in flight, so the GPU never waits for the host's check.  Every launch
   // that does not find the record done attempts a step, and the record is
-  // done after at most max_steps + 2 attempts.
+  // done after at most max_steps + 2 attempts.  The first batch is launched
+  // directly (short solves end there); later batches replay one captured
+  // HIP graph of kGraphSteps step launches, which costs the host a fraction
+  // of a launch per step.
   int* flags = pinned_flags();
   if (!flags) return GX_ENOMEM;
+  hipGraphExec_t gexec = nullptr;
+  {
+    hipStream_t cap = capture_stream();
+    hipGraph_t graph = nullptr;
+    bool ok = cap && hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) ==
+                         hipSuccess;
+    if (ok) {
+      for (int k = 0; k < kGraphSteps; ++k)
+        hipLaunchKernelGGL((k_sh_step<M, PID, E>), dim3(blocks),
+                           dim3(kShBlock), 0, cap, pot->dev, A);
+      ok = hipStreamEndCapture(cap, &graph) == hipSuccess && graph;
+    }
+    if (ok && hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) !=
+                  hipSuccess)
+      gexec = nullptr;
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();  // a failed capture leaves the direct launches
+  }
   hipEvent_t ev[2];
   if (hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) != hipSuccess)
     return GX_ELAUNCH;
@@ -454,11 +487,16 @@ int run(const gx_pot* pot, SArgs& A, bool first_step, hipStream_t stream) {
   int rc = GX_OK;
   auto issue = [&](int slot) {
     flags[slot] = 0;
-    for (int k = 0; k < chunk; ++k)
-      hipLaunchKernelGGL((k_sh_step<M, PID, E>), dim3(blocks), dim3(kShBlock),
-                         0, stream, pot->dev, A);
-    launched += chunk;
-    chunk = chunk < 64 ? 2 * chunk : 64;
+    if (gexec && launched > 0) {
+      if (hipGraphLaunch(gexec, stream) != hipSuccess) rc = GX_ELAUNCH;
+      launched += kGraphSteps;
+    } else {
+      for (int k = 0; k < chunk; ++k)
+        hipLaunchKernelGGL((k_sh_step<M, PID, E>), dim3(blocks),
+                           dim3(kShBlock), 0, stream, pot->dev, A);
+      launched += chunk;
+      chunk = chunk < 64 ? 2 * chunk : 64;
+    }
     if (hipMemcpyAsync(&flags[slot], &A.ctl->done, sizeof(int),
                        hipMemcpyDeviceToHost, stream) != hipSuccess ||
         hipEventRecord(ev[slot], stream) != hipSuccess)
@@ -480,6 +518,7 @@ int run(const gx_pot* pot, SArgs& A, bool first_step, hipStream_t stream) {
   (void)hipEventSynchronize(ev[slot ^ 1]);
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
+  if (gexec) (void)hipGraphExecDestroy(gexec);
   if (rc != GX_OK) return rc;
   hipLaunchKernelGGL(k_sh_finish, dim3(blocks), dim3(kShBlock), 0, stream, A);
   return check(hipGetLastError());
@@ -495,14 +534,19 @@ extern "C" {
 int64_t gx_integrate_shared_workspace_bytes(int64_t n) {
   if (n < 0) return 0;
   const int64_t blocks = (n + kShBlock - 1) / kShBlock;
-  return 512 + 16 * blocks + 8 * (12 * n) + 8 * (3 * n);
+  return 512 + 16 * blocks + 8 * (2 * kRow * n);
 }
 
-int gx_integrate_shared(const gx_pot* pot, const gx_solver_cfg* cfg,
-                        const double* w0, double t0, double t1,
-                        const double* ts, int32_t T, int64_t n, double* out,
-                        int32_t* status, int64_t* nsteps, int64_t* naccept,
-                        void* workspace, void* stream) {
+}  // extern "C"
+
+namespace {
+
+int shared_solve(const gx_pot* pot, const gx_solver_cfg* cfg,
+                 const double* w0, double t0, double t1, const double* ts,
+                 int32_t T, int64_t n, double* out, int32_t* status,
+                 int64_t* nsteps, int64_t* naccept, double* rec,
+                 int64_t rec_cap, int64_t* nrec, void* workspace,
+                 void* stream) {
   if (!pot || !cfg || n < 0 || T < 1) return GX_EINVAL;
   if (n == 0) return GX_OK;
   if (!w0 || !ts || !out || !workspace) return GX_EINVAL;
@@ -551,8 +595,10 @@ int gx_integrate_shared(const gx_pot* pot, const gx_solver_cfg* cfg,
   A.counter = reinterpret_cast<unsigned int*>(ws + 256);
   A.part = reinterpret_cast<double*>(ws + 512);
   A.y = A.part + 2 * blocks;
-  A.f0 = A.y + 12 * n;
-  const bool first_step = !(cfg->dt0 > 0.0) && t0 != t1;
+  A.rec = rec;
+  A.rec_cap = rec_cap;
+  A.nrec = reinterpret_cast<long long*>(nrec);
+  const bool first_step = !(cfg->dt0 > 0.0) && !(A.dtmin > 0.0) && t0 != t1;
   if (!(cfg->dt0 > 0.0) && !pid) return GX_EINVAL;
 
   hipStream_t s = (hipStream_t)stream;
@@ -573,6 +619,31 @@ int gx_integrate_shared(const gx_pot* pot, const gx_solver_cfg* cfg,
                       : run<Dopri5, true, Full>(pot, A, first_step, s);
   return pot->basic ? run<Dopri5, false, Basic>(pot, A, first_step, s)
                     : run<Dopri5, false, Full>(pot, A, first_step, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gx_integrate_shared(const gx_pot* pot, const gx_solver_cfg* cfg,
+                        const double* w0, double t0, double t1,
+                        const double* ts, int32_t T, int64_t n, double* out,
+                        int32_t* status, int64_t* nsteps, int64_t* naccept,
+                        void* workspace, void* stream) {
+  return shared_solve(pot, cfg, w0, t0, t1, ts, T, n, out, status, nsteps,
+                      naccept, nullptr, 0, nullptr, workspace, stream);
+}
+
+int gx_integrate_shared_dense(const gx_pot* pot, const gx_solver_cfg* cfg,
+                              const double* w0, double t0, double t1,
+                              const double* ts, int32_t T, int64_t n,
+                              double* out, int32_t* status, int64_t* nsteps,
+                              int64_t* naccept, double* rec, int64_t rec_cap,
+                              int64_t* nrec, void* workspace, void* stream) {
+  if (!rec || !nrec || rec_cap < 1) return n == 0 ? GX_OK : GX_EINVAL;
+  if (n > 0 && rec_cap > ((int64_t)1 << 40) / n) return GX_EINVAL;
+  return shared_solve(pot, cfg, w0, t0, t1, ts, T, n, out, status, nsteps,
+                      naccept, rec, rec_cap, nrec, workspace, stream);
 }
 
 }  // extern "C"

@@ -228,15 +228,151 @@ constexpr Tab<7> dopri5_tab() {
   return t;
 }
 
+// The tableau with one more stage at (t0 + h, y1): row b, c = 1, no weight
+// in the solution or the error.  Its acceleration a(q1, t1) is the next
+// step's first stage (first same as last) and the 14th increment that
+// Dopri8's dense output reads.
+template <int S>
+constexpr Tab<S + 1> with_end_stage(const Tab<S>& t) {
+  Tab<S + 1> r{};
+  for (int i = 0; i < S; ++i) {
+    r.c[i] = t.c[i];
+    r.b[i] = t.b[i];
+    r.e[i] = t.e[i];
+    for (int j = 0; j < S; ++j) r.a[i][j] = t.a[i][j];
+  }
+  r.c[S] = 1.0;
+  for (int j = 0; j < S; ++j) r.a[S][j] = t.b[j];
+  return r;
+}
+
+// Dense output over the SF stage slots of a step whose last slot is the
+// end stage, in Runge-Kutta-Nystrom form.  With first-order weights w_l(th)
+// (y(t0 + th h) = y0 + sum_l w_l(th) h f_l, sum_l w_l = th):
+//   p(th) = p + h th sum_l Wp_l(th) g_l,          Wp_l = w_l / th
+//   q(th) = q + th (h p) + h^2 th sum_l Wq_l(th) g_l,
+//                                     Wq_l = sum_i Wp_i a_il
+// because the velocity stage i is p + h sum_l a_il g_l.  Polynomials are
+// stored in Horner order (th^(D-1) first).
+template <int SF, int D>
+struct Dense {
+  double wp[SF][D];
+  double wq[SF][D];
+  bool np[SF], nq[SF];  // rows that are not identically zero
+};
+
+template <int SF, int D>
+constexpr Dense<SF, D> make_dense(const double (&wp)[SF][D],
+                                  const Tab<SF>& t) {
+  Dense<SF, D> r{};
+  for (int l = 0; l < SF; ++l)
+    for (int d = 0; d < D; ++d) {
+      r.wp[l][d] = wp[l][d];
+      double s = 0.0;
+      for (int i = 0; i < SF; ++i) s += wp[i][d] * t.a[i][l];
+      r.wq[l][d] = s;
+    }
+  for (int l = 0; l < SF; ++l) {
+    bool a = false, b = false;
+    for (int d = 0; d < D; ++d) {
+      a = a || r.wp[l][d] != 0.0;
+      b = b || r.wq[l][d] != 0.0;
+    }
+    r.np[l] = a;
+    r.nq[l] = b;
+  }
+  return r;
+}
+
+// Dopri8's interpolant (diffrax 0.7.0 Dopri8 dense output, a continuous
+// extension of RK8(7)13M over its 13 stages and the end stage): w_l(th) =
+// th * (quintic in th).  Stages 2-5 carry no weight; stage 1's row follows
+// from sum_l w_l = th.
+constexpr Dense<14, 6> dopri8_dense() {
+  double w[14][6] = {};
+  const double rows[9][6] = {
+      {-39.6107919852202505218, 116.4422149550342161651,
+       -121.4999627731334642623, 52.2273532792945524050,
+       -7.6142658045872677172, 0},
+      {20.3761213808791436958, -67.1451318825957197185,
+       83.1721004639847717481, -46.8919164181093621583,
+       10.7281392630428866124, 0},
+      {7.3347098826795362023, -16.5672243527496524646, 9.5724507555993664382,
+       -0.1890893225010595467, 0.5526637063753648783, 0},
+      {32.8801774352459155182, -89.9916014847245016028,
+       87.8406057677205645007, -35.7075975946222072821,
+       4.2186562625665153803, 0},
+      {-10.1588990526426760954, 22.6237489648532849093,
+       -17.4152107770762969005, 6.2736448083240352160,
+       -0.6627209125361597559, 0},
+      {-12.5401268098782561200, 32.2362340167355370113,
+       -28.5903289514790976966, 10.3160881272450748458,
+       -1.2636789001135462218, 0},
+      {29.5553001484516038033, -82.1020315488359848644,
+       81.6630950584341412934, -34.7650769866611817349,
+       5.4106037898590422230, 0},
+      {-41.7923486424390588923, 116.2662185791119533462,
+       -114.9375291377009418170, 47.7457971078225540396,
+       -7.0321379067945741781, 0},
+      {20.3006925822100825485, -53.9020777466385396792,
+       50.2558364226176017553, -19.0082099341608028453,
+       2.3537586759714983486, 0}};
+  for (int r = 0; r < 9; ++r)
+    for (int d = 0; d < 6; ++d) w[5 + r][d] = rows[r][d];
+  for (int d = 0; d < 6; ++d) {
+    double s = (d == 5) ? 1.0 : 0.0;
+    for (int i = 5; i < 14; ++i) s -= w[i][d];
+    w[0][d] = s;
+  }
+  return make_dense<14, 6>(w, with_end_stage(dopri8_tab()));
+}
+
+// Dopri5's interpolant (diffrax's, and the one scipy's RK45 uses): the
+// quartic through y0, y1, the end slopes f0 = k_1, f1 = k_7 and Shampine's
+// (1986) midpoint value y0 + sum_i m_i k_i.  Written over the increments,
+// w_i(th) / th = a_i th^3 + b_i th^2 + c_i th + [i = 1] with
+//   a_i = 2([i=7] - [i=1]) - 8 b_i + 16 m_i
+//   b_i = 5 [i=1] - 3 [i=7] + 14 b_i - 32 m_i
+//   c_i = [i=7] - 4 [i=1] - 5 b_i + 16 m_i
+// (the y0 terms of the quartic cancel).  Its 7th stage is already at y1.
+constexpr Dense<7, 4> dopri5_dense() {
+  const Tab<7> t = dopri5_tab();
+  const double m[7] = {6025192743.0 / 30085553152.0 / 2,
+                       0,
+                       51252292925.0 / 65400821598.0 / 2,
+                       -2691868925.0 / 45128329728.0 / 2,
+                       187940372067.0 / 1594534317056.0 / 2,
+                       -1776094331.0 / 19743644256.0 / 2,
+                       11237099.0 / 235043384.0 / 2};
+  double w[7][4] = {};
+  for (int i = 0; i < 7; ++i) {
+    const double first = (i == 0) ? 1.0 : 0.0, last = (i == 6) ? 1.0 : 0.0;
+    w[i][0] = 2.0 * (last - first) - 8.0 * t.b[i] + 16.0 * m[i];
+    w[i][1] = 5.0 * first - 3.0 * last + 14.0 * t.b[i] - 32.0 * m[i];
+    w[i][2] = last - 4.0 * first - 5.0 * t.b[i] + 16.0 * m[i];
+    w[i][3] = first;
+  }
+  return make_dense<7, 4>(w, t);
+}
+
+// S: stages of the method; SF: stage slots of a step including the end
+// stage (Dopri5's last stage is its end stage); kf: the Runge-Kutta-Nystrom
+// form over the SF slots; dn: the dense output over them.
 struct Dopri8 {
   static constexpr int S = 13;
+  static constexpr int SF = 14;
   static constexpr int order = 8;
   static constexpr Rkn<13> k = make_rkn(dopri8_tab());
+  static constexpr Rkn<14> kf = make_rkn(with_end_stage(dopri8_tab()));
+  static constexpr Dense<14, 6> dn = dopri8_dense();
 };
 struct Dopri5 {
   static constexpr int S = 7;
+  static constexpr int SF = 7;
   static constexpr int order = 5;
   static constexpr Rkn<7> k = make_rkn(dopri5_tab());
+  static constexpr Rkn<7> kf = make_rkn(dopri5_tab());
+  static constexpr Dense<7, 4> dn = dopri5_dense();
 };
 
 }  // namespace gx

@@ -147,29 +147,21 @@ class Orbit:
 
 
 class OrbitInterpolant:
-    """Dense orbit: evaluates the solution at any time by integrating from the
-    nearest stored state at or before (towards the start) that time with the
-    same solver.  Accurate to the solver's tolerance."""
+    """Dense orbit (galax's Orbit interpolant, orbit/interp.py:42-145, over
+    diffrax's DenseInterpolation): the solve's accepted-step records stay on
+    the device and every query is the method's dense output over the step
+    holding it, one GPU lane per (orbit, time) -- the same polynomial the
+    solve's own saves come from."""
 
-    def __init__(self, pot, solver, dt0, ts, w):
-        self.pot = pot
-        self.solver = solver
-        self.dt0 = dt0
-        self.ts = np.asarray(ts, dtype=np.float64)  # (T,)
-        self.w = w  # (n, T, 6) numpy
+    def __init__(self, dense, batch=None):
+        self.dense = dense  # DenseSolution
+        self.batch = batch
 
     def __call__(self, t):
         t = np.atleast_1d(np.asarray(ustrip(t, "Myr"), dtype=np.float64))
-        n, T = self.w.shape[0], self.w.shape[1]
-        fwd = self.ts[-1] >= self.ts[0]
-        key = self.ts if fwd else -self.ts
-        tk = t if fwd else -t
-        idx = np.clip(np.searchsorted(key, tk, side="right") - 1, 0, T - 1)
-        out = np.empty((n, t.size, 6))
-        for j, (tj, ij) in enumerate(zip(t, idx)):
-            w0 = self.w[:, ij, :]
-            res = integrate_orbits(self.pot, w0, self.ts[ij], tj, solver=self.solver, dt0=self.dt0)
-            out[:, j, :] = res.out[:, 0, :].cpu().numpy()
+        out = self.dense.evaluate(t).cpu().numpy()
+        if self.batch is not None:
+            out = out.reshape(*self.batch, t.size, 6)
         return PhaseSpaceCoordinate(q=out[..., :3], p=out[..., 3:], t=t)
 
 
@@ -207,7 +199,7 @@ class Integrator:
         adaptive = self.dynamics_solver.stepsize_controller.__class__.__name__ == "PIDController"
         return scalar and adaptive and int(np.shape(w)[0]) > 1
 
-    def _run(self, pot, w, t0, t1, ts=None):
+    def _run(self, pot, w, t0, t1, ts=None, dense=False):
         kw = self.diffeq_kw
         return integrate_orbits(
             pot,
@@ -219,6 +211,7 @@ class Integrator:
             dt0=kw.get("dt0"),
             max_steps=kw.get("max_steps", "default"),
             shared=self._shared(w, t0, t1) and (ts is None or np.ndim(ts) == 1),
+            dense=dense,
         )
 
     def __call__(self, field, w0, t0, t1, *, saveat=None, dense=False):
@@ -242,6 +235,12 @@ class Integrator:
 default_integrator = Integrator()
 
 
+# evaluate_orbit serves more than _MANY_SAVES save times of at most
+# _FEW_ORBITS orbits from dense records
+_MANY_SAVES = 256
+_FEW_ORBITS = 64
+
+
 def evaluate_orbit(pot, w0, t, *, integrator: Integrator | None = None, dense: bool = False) -> Orbit:
     """Integrate orbits in `pot`, returning them at the times `t`.
 
@@ -263,22 +262,22 @@ def evaluate_orbit(pot, w0, t, *, integrator: Integrator | None = None, dense: b
         if np.any(tw0 != t[0]):
             r1 = integrator._run(pot, w, tw0, np.full(n, t[0]))
             w = r1.out[:, 0, :]
-    # stage 2: t[0] -> t[-1], saving at t
-    r2 = integrator._run(pot, w, t[0], t[-1], ts=t)
-    out = r2.out
+    # stage 2: t[0] -> t[-1], saving at t (and keeping the dense records)
     T = t.shape[0]
+    if T > _MANY_SAVES and n <= _FEW_ORBITS:
+        # many save times on few orbits (a stream's progenitor): rather than
+        # one lane serving every save in turn, keep the step records and
+        # evaluate the same dense output at all the times in parallel
+        r2 = integrator._run(pot, w, t[0], t[-1], dense=True)
+        out = r2.dense.evaluate(t)
+        out[:, -1, :] = r2.out[:, 0, :]  # the end point as the solve left it
+    else:
+        r2 = integrator._run(pot, w, t[0], t[-1], ts=t, dense=dense)
+        out = r2.out
     if not is_torch:
         out = out.cpu().numpy()
     out = out.reshape(*batch, T, 6)
-    interp = None
-    if dense:
-        interp = OrbitInterpolant(
-            pot,
-            integrator.dynamics_solver,
-            integrator.diffeq_kw.get("dt0"),
-            t,
-            out.reshape(n, T, 6) if not is_torch else out.reshape(n, T, 6).cpu().numpy(),
-        )
+    interp = OrbitInterpolant(r2.dense, batch) if dense else None
     status = r2.status if is_torch else r2.status.cpu().numpy().reshape(batch)
     nsteps = r2.nsteps if is_torch else r2.nsteps.cpu().numpy().reshape(batch)
     return Orbit(out[..., :3], out[..., 3:], t, interpolant=interp, status=status, nsteps=nsteps)

@@ -158,6 +158,40 @@ class OrbitSolver:
         return NBodySolution(ts=ts if ts is not None else t1v, ys=ys, status=st, nsteps=ns, naccept=na)
 
 
+class DenseSolution:
+    """The dense solution of a solve on the device: every accepted step's
+    record (start time, signed step, q, v, stage accelerations; device
+    tensors) and the method's dense output over it (gx_dense_eval), as
+    diffrax's DenseInterpolation keeps and evaluates them."""
+
+    def __init__(self, method: int, rec, nrec, cap: int):
+        self.method = method
+        self.rec = rec  # (n, cap, GX_DENSE_RECORD) float64
+        self.nrec = nrec  # (n,) int64
+        self.cap = cap
+
+    @property
+    def n(self) -> int:
+        return int(self.nrec.shape[0])
+
+    def evaluate(self, t):
+        """States (n, T, 6) at times t: (T,) for every orbit or (n, T) per
+        orbit; NaN outside an orbit's integrated interval."""
+        torch = _lib.require_gpu()
+        td = _dev(torch, t, torch.float64)
+        scalar = td.ndim == 0
+        td = td.reshape(1) if scalar else td
+        per_orbit = 1 if td.ndim == 2 else 0
+        if per_orbit and td.shape[0] != self.n:
+            raise ValueError("per-orbit times must have shape (n, T)")
+        T = int(td.shape[-1])
+        out = torch.empty((self.n, T, 6), dtype=torch.float64, device="cuda")
+        rc = _lib.lib().gx_dense_eval(self.method, _lib.ptr(self.rec), self.cap, _lib.ptr(self.nrec), self.n,
+                                      _lib.ptr(td), per_orbit, T, _lib.ptr(out), _lib.stream_ptr())
+        _lib.check(rc, "gx_dense_eval")
+        return out
+
+
 @dataclass
 class IntegrationResult:
     """Device-side result of one gx_integrate call (torch tensors)."""
@@ -166,6 +200,7 @@ class IntegrationResult:
     status: object  # (n,) int32
     nsteps: object  # (n,) int64 attempted steps
     naccept: object  # (n,) int64 accepted steps
+    dense: object = None  # DenseSolution when requested
 
     @property
     def q(self):
@@ -200,6 +235,8 @@ def integrate_orbits(
     workspace=None,
     stream=None,
     shared: bool = False,
+    dense: bool | int = False,
+    _records=None,
 ) -> IntegrationResult:
     """Integrate n orbits on the GPU.
 
@@ -208,7 +245,15 @@ def integrate_orbits(
     without: the final state only).  Every orbit takes its own step sequence
     unless `shared`: then the batch is one system with one step sequence
     (scalar t0, t1 and shared or no save times), as galax's Integrator
-    solves a batch with scalar times.  Returns device tensors."""
+    solves a batch with scalar times.  `dense` (True, or a record capacity
+    per orbit) also keeps every accepted step's dense-output record
+    (`result.dense`, a DenseSolution); a capacity too small for an orbit is
+    raised until it fits.  Returns device tensors."""
+    if dense:
+        return _integrate_dense(pot, w0, t0, t1, ts=ts, T=T, solver=solver, dt0=dt0, max_steps=max_steps,
+                                ts_mode=ts_mode, order=order, validate_order=validate_order, out=out,
+                                workspace=workspace, stream=stream, shared=shared,
+                                cap=None if dense is True else int(dense))
     torch = _lib.require_gpu()
     solver = solver or OrbitSolver()
     cfg = solver.cfg(dt0=dt0, max_steps=max_steps)
@@ -217,7 +262,7 @@ def integrate_orbits(
         raise ValueError(f"w0 must have shape (n, 6), got {tuple(w0d.shape)}")
     n = int(w0d.shape[0])
     if shared:
-        return _integrate_shared(torch, pot, cfg, w0d, n, t0, t1, ts, T, out, workspace, stream)
+        return _integrate_shared(torch, pot, cfg, w0d, n, t0, t1, ts, T, out, workspace, stream, _records)
     t0d = _dev(torch, t0, torch.float64).reshape(-1)
     t1d = _dev(torch, t1, torch.float64).reshape(-1)
     if t0d.numel() == 1 and t1d.numel() == 1:
@@ -239,19 +284,30 @@ def integrate_orbits(
         mode = _abi.GX_TS_LINSPACE
     else:
         mode, T = _abi.GX_TS_FINAL, 1
+    if tsd is not None and tsd.shape[-1] > 1:
+        # saves are served in order along the integration (diffrax requires
+        # ordered SaveAt(ts) too)
+        step = tsd[..., 1:] - tsd[..., :-1]
+        sgn = torch.sign(t1d - t0d).reshape(-1, 1)
+        if bool(torch.any(step.reshape(-1 if tsd.ndim == 2 else 1, T - 1) * sgn < 0)):
+            raise ValueError("save times must be ordered in the direction of integration")
     if ts_mode is not None:
         mode = ts_mode
     if T < 1:
         raise ValueError("need at least one save time")
     if out is None:
         out = torch.empty((n, T, 6), dtype=torch.float64, device="cuda")
+    else:
+        _check_out(torch, out, n, T)
     status = torch.empty((n,), dtype=torch.int32, device="cuda")
     nsteps = torch.empty((n,), dtype=torch.int64, device="cuda")
     naccept = torch.empty((n,), dtype=torch.int64, device="cuda")
     lib = _lib.lib()
+    nbytes = int(lib.gx_integrate_workspace_bytes(n))
     if workspace is None:
-        nbytes = int(lib.gx_integrate_workspace_bytes(n))
         workspace = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device="cuda")
+    else:
+        _check_workspace(torch, workspace, nbytes)
     orderd = None
     if order is not None:
         orderd = _dev(torch, order, torch.int64)
@@ -261,30 +317,85 @@ def integrate_orbits(
             torch.equal(torch.sort(orderd).values, torch.arange(n, device="cuda", dtype=torch.int64))
         ):
             raise ValueError("order must be a permutation of range(n)")
-    rc = lib.gx_integrate(
-        pot._handle(),
-        __import__("ctypes").addressof(cfg),
-        _lib.ptr(w0d),
-        _lib.ptr(t0d),
-        _lib.ptr(t1d),
-        t_stride,
-        _lib.ptr(tsd),
-        T,
-        mode,
-        n,
-        _lib.ptr(orderd),
-        _lib.ptr(out),
-        _lib.ptr(status),
-        _lib.ptr(nsteps),
-        _lib.ptr(naccept),
-        _lib.ptr(workspace),
-        stream if stream is not None else _lib.stream_ptr(),
-    )
+    args = [pot._handle(), __import__("ctypes").addressof(cfg), _lib.ptr(w0d), _lib.ptr(t0d), _lib.ptr(t1d),
+            t_stride, _lib.ptr(tsd), T, mode, n, _lib.ptr(orderd), _lib.ptr(out), _lib.ptr(status), _lib.ptr(nsteps),
+            _lib.ptr(naccept)]
+    tail = [_lib.ptr(workspace), stream if stream is not None else _lib.stream_ptr()]
+    if _records is None:
+        rc = lib.gx_integrate(*args, *tail)
+    else:
+        rec, cap, nrec = _records
+        rc = lib.gx_integrate_dense(*args, _lib.ptr(rec), cap, _lib.ptr(nrec), *tail)
     _lib.check(rc, "gx_integrate")
     return IntegrationResult(out=out, status=status, nsteps=nsteps, naccept=naccept)
 
 
-def _integrate_shared(torch, pot, cfg, w0d, n, t0, t1, ts, T, out, workspace, stream) -> IntegrationResult:
+def _integrate_dense(pot, w0, t0, t1, *, cap, shared, **kw) -> IntegrationResult:
+    """integrate_orbits with dense records: the solve runs with `cap`
+    records per orbit (default 1024) and again with eight times more while
+    an orbit's accepted steps do not fit."""
+    torch = _lib.require_gpu()
+    solver = kw.get("solver") or OrbitSolver()
+    fixed_cap = cap is not None
+    if cap is None:
+        # a first capacity within about 256 MB of records
+        n = int(np.shape(w0)[0]) if not hasattr(w0, "shape") else int(w0.shape[0])
+        cap = int(min(65536, max(1024, 256e6 / (8 * _abi.GX_DENSE_RECORD * max(n, 1)))))
+    cap = int(cap)
+    if cap < 1:
+        raise ValueError("dense record capacity must be positive")
+    cfg = solver.cfg(dt0=kw.get("dt0"), max_steps=kw.get("max_steps", "default"))
+    limit = cfg.max_steps if cfg.max_steps > 0 else 10_000_000
+    while True:
+        res = _integrate_records(torch, pot, w0, t0, t1, cap=cap, shared=shared, **kw)
+        overflow = (res.status == _abi.GX_MAX_STEPS_REACHED) & (res.naccept > cap)
+        if fixed_cap or not bool(overflow.any()) or cap >= limit:
+            return res
+        cap = min(8 * cap, limit)
+
+
+def _integrate_records(torch, pot, w0, t0, t1, *, cap, shared, ts=None, T=None, solver=None, dt0=None,
+                       max_steps="default", ts_mode=None, order=None, validate_order=True, out=None,
+                       workspace=None, stream=None) -> IntegrationResult:
+    solver = solver or OrbitSolver()
+    w0d = _dev(torch, w0, torch.float64)
+    n = int(w0d.shape[0])
+    rec = torch.empty((n, cap, _abi.GX_DENSE_RECORD), dtype=torch.float64, device="cuda")
+    nrec = torch.zeros((n,), dtype=torch.int64, device="cuda")
+    res = integrate_orbits(pot, w0d, t0, t1, ts=ts, T=T, solver=solver, dt0=dt0, max_steps=max_steps,
+                           ts_mode=ts_mode, order=order, validate_order=validate_order, out=out,
+                           workspace=workspace, stream=stream, shared=shared, _records=(rec, cap, nrec))
+    res.dense = DenseSolution(solver.cfg(dt0=dt0).method, rec, nrec, cap)
+    return res
+
+
+def _check_out(torch, out, n: int, T: int):
+    """A caller's output buffer must be exactly what the kernels write:
+    float64 (n, T, 6), contiguous, on the GPU."""
+    if not isinstance(out, torch.Tensor):
+        raise ValueError("out must be a torch tensor")
+    if tuple(out.shape) != (n, T, 6):
+        raise ValueError(f"out must have shape {(n, T, 6)}, got {tuple(out.shape)}")
+    if out.dtype != torch.float64 or out.device.type != "cuda" or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float64 CUDA tensor")
+
+
+def _check_workspace(torch, ws, nbytes: int):
+    """A caller's workspace must be a contiguous CUDA buffer of at least
+    nbytes, 256-byte aligned (the library's counters and records sit at its
+    start)."""
+    if not isinstance(ws, torch.Tensor):
+        raise ValueError("workspace must be a torch tensor")
+    if ws.numel() * ws.element_size() < nbytes:
+        raise ValueError(f"workspace holds {ws.numel() * ws.element_size()} bytes; this call needs {nbytes}")
+    if ws.device.type != "cuda" or not ws.is_contiguous():
+        raise ValueError("workspace must be a contiguous CUDA tensor")
+    if ws.data_ptr() % 256:
+        raise ValueError("workspace must be 256-byte aligned")
+
+
+def _integrate_shared(torch, pot, cfg, w0d, n, t0, t1, ts, T, out, workspace, stream,
+                      records=None) -> IntegrationResult:
     t0v = np.asarray(t0.cpu() if isinstance(t0, torch.Tensor) else t0, dtype=np.float64)
     t1v = np.asarray(t1.cpu() if isinstance(t1, torch.Tensor) else t1, dtype=np.float64)
     if t0v.size != 1 or t1v.size != 1:
@@ -298,30 +409,28 @@ def _integrate_shared(torch, pot, cfg, w0d, n, t0, t1, ts, T, out, workspace, st
     T = int(tsd.shape[0])
     if T < 1:
         raise ValueError("need at least one save time")
+    if T > 1 and bool(torch.any((tsd[1:] - tsd[:-1]) * float(np.sign(t1f - t0f)) < 0)):
+        raise ValueError("save times must be ordered in the direction of integration")
     if out is None:
         out = torch.empty((n, T, 6), dtype=torch.float64, device="cuda")
+    else:
+        _check_out(torch, out, n, T)
     status = torch.empty((n,), dtype=torch.int32, device="cuda")
     nsteps = torch.empty((n,), dtype=torch.int64, device="cuda")
     naccept = torch.empty((n,), dtype=torch.int64, device="cuda")
     lib = _lib.lib()
     nbytes = int(lib.gx_integrate_shared_workspace_bytes(n))
-    if workspace is None or workspace.numel() * workspace.element_size() < nbytes:
+    if workspace is None:
         workspace = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device="cuda")
-    rc = lib.gx_integrate_shared(
-        pot._handle(),
-        __import__("ctypes").addressof(cfg),
-        _lib.ptr(w0d),
-        t0f,
-        t1f,
-        _lib.ptr(tsd),
-        T,
-        n,
-        _lib.ptr(out),
-        _lib.ptr(status),
-        _lib.ptr(nsteps),
-        _lib.ptr(naccept),
-        _lib.ptr(workspace),
-        stream if stream is not None else _lib.stream_ptr(),
-    )
+    else:
+        _check_workspace(torch, workspace, nbytes)
+    args = [pot._handle(), __import__("ctypes").addressof(cfg), _lib.ptr(w0d), t0f, t1f, _lib.ptr(tsd), T, n,
+            _lib.ptr(out), _lib.ptr(status), _lib.ptr(nsteps), _lib.ptr(naccept)]
+    tail = [_lib.ptr(workspace), stream if stream is not None else _lib.stream_ptr()]
+    if records is None:
+        rc = lib.gx_integrate_shared(*args, *tail)
+    else:
+        rec, cap, nrec = records
+        rc = lib.gx_integrate_shared_dense(*args, _lib.ptr(rec), cap, _lib.ptr(nrec), *tail)
     _lib.check(rc, "gx_integrate_shared")
     return IntegrationResult(out=out, status=status, nsteps=nsteps, naccept=naccept)

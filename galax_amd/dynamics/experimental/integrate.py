@@ -15,7 +15,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from ...units import ustrip
-from .._orbit import HamiltonianField, OrbitInterpolant
+from .._orbit import HamiltonianField
 from .._solver import Dopri8, OrbitSolver, PIDController, integrate_orbits
 
 __all__ = ["Solution", "default_solver", "integrate_orbit", "parse_t0_t1_saveat"]
@@ -84,37 +84,26 @@ def integrate_orbit(pot, qp0, t0=None, t1=None, *, saveat=None, solver: OrbitSol
     w0 = np.concatenate([np.atleast_2d(q), np.atleast_2d(p)], -1)
     n = w0.shape[0]
     t0v, t1v, ts = parse_t0_t1_saveat(t0, t1, saveat, dense=dense)
-    dense_ts = None
     if dense:
-        # the stored states of the dense solution: 64 knots per orbit
-        dense_ts = np.linspace(np.broadcast_to(t0v, (n,)), np.broadcast_to(t1v, (n,)), 64, axis=-1)
-        ts = dense_ts
+        res = integrate_orbits(pot, w0, np.broadcast_to(t0v, (n,)), np.broadcast_to(t1v, (n,)), solver=solver,
+                               dt0=kw.get("dt0"), max_steps=kw.get("max_steps", "default"), dense=True)
+        status = res.status.cpu().numpy()
+        nsteps = res.nsteps.cpu().numpy()
+        sol_dense = res.dense
+
+        def interpolation(t, _d=sol_dense):
+            o = _d.evaluate(np.atleast_1d(np.asarray(ustrip(t, "Myr"), dtype=np.float64))).cpu().numpy()
+            qs, ps = o[..., :3], o[..., 3:]
+            return (qs[0], ps[0]) if single else (qs, ps)
+
+        return Solution(t0=t0v, t1=t1v, ts=None, ys=None, result=status[0] if single else status,
+                        num_steps=nsteps[0] if single else nsteps, interpolation=interpolation)
     res = integrate_orbits(pot, w0, np.broadcast_to(t0v, (n,)), np.broadcast_to(t1v, (n,)),
                            ts=np.broadcast_to(ts, (n, ts.shape[-1])) if ts.ndim == 2 else ts,
                            solver=solver, dt0=kw.get("dt0"), max_steps=kw.get("max_steps", "default"))
     out = res.out.cpu().numpy()  # (n, T, 6)
     status = res.status.cpu().numpy()
     nsteps = res.nsteps.cpu().numpy()
-    if dense:
-        # one interpolant over the batch when the knots are shared
-        if np.ndim(t0v) == 0 and np.ndim(t1v) == 0:
-            groups = [(np.arange(n), OrbitInterpolant(pot, solver, kw.get("dt0"), dense_ts[0], out))]
-        else:
-            groups = [(np.array([i]), OrbitInterpolant(pot, solver, kw.get("dt0"), dense_ts[i], out[i : i + 1]))
-                      for i in range(n)]
-
-        def interpolation(t, _groups=groups, _n=n):
-            nt = np.atleast_1d(np.asarray(ustrip(t, "Myr"))).size
-            qs = np.empty((_n, nt, 3))
-            ps = np.empty((_n, nt, 3))
-            for idx, f in _groups:
-                c = f(t)
-                qs[idx] = np.asarray(c.q)
-                ps[idx] = np.asarray(c.p)
-            return (qs[0], ps[0]) if single else (qs, ps)
-
-        return Solution(t0=t0v, t1=t1v, ts=None, ys=None, result=status[0] if single else status,
-                        num_steps=nsteps[0] if single else nsteps, interpolation=interpolation)
     ys = (out[..., :3], out[..., 3:])
     if single:
         ys = (ys[0][0], ys[1][0])

@@ -4,8 +4,13 @@ Unit: one attempted Runge-Kutta step of one orbit (an orbit-step).
 Counting rule: add, sub, mul = 1; fma = 2; div, sqrt, rsqrt, log1p, sincos
 = 1 algorithmic operation each.
 
-  F_step = S * F_rhs + 2 * 6 * (nnz(A) + nnz(b) + nnz(b - b_hat)) + F_ctrl
-  F_save = S * F_rhs + 2 * 6 * (nnz(A) + nnz(b))      (a step without error)
+  F_step  = S * F_rhs + 2 * 6 * (nnz(A) + nnz(b) + nnz(b - b_hat)) + F_ctrl
+  F_dense = 2 * (D - 1) * (n_q + n_p) + 2 * 3 * (n_q + n_p) + 24
+            (one save inside a step from the method's dense output: Horner
+            evaluation of the n_q position and n_p velocity weight
+            polynomials of degree D - 1, their combination with the stage
+            accelerations, the final three FMAs per component, and th)
+  F_save  = S * F_rhs + 2 * 6 * (nnz(A) + nnz(b))      (a step without error)
 
 nnz values are those of the implemented tableaux (checked against the
 oracle's tableau in the tests).  F_rhs counts the closed-form field as
@@ -27,6 +32,10 @@ NNZ = {
     "dopri8": (13, 59, 9, 9),
     "dopri5": (7, 20, 5, 6),
 }
+# dense output: (D, n_q, n_p) -- polynomial length, nonzero position and
+# velocity weight rows over the step's stage slots (13 stages and the end
+# stage for Dopri8, 7 for Dopri5; checked against the oracle in the tests)
+NNZ_DENSE = {"dopri8": (6, 11, 10), "dopri5": (4, 5, 6)}
 F_CTRL = 60
 F_RHS = {"kepler": 9, "hernquist": 11, "plummer": 11, "nfw": 15, "miyamoto_nagai": 22, "shared_radius": 8}
 F_RHS_MW = F_RHS["shared_radius"] + F_RHS["miyamoto_nagai"] + F_RHS["nfw"] + 2 * F_RHS["hernquist"]
@@ -35,6 +44,11 @@ F_RHS_MW = F_RHS["shared_radius"] + F_RHS["miyamoto_nagai"] + F_RHS["nfw"] + 2 *
 def f_step(method: str = "dopri8", f_rhs: int = F_RHS_MW) -> int:
     S, a, b, e = NNZ[method]
     return S * f_rhs + 12 * (a + b + e) + F_CTRL
+
+
+def f_dense(method: str = "dopri8") -> int:
+    D, nq, npv = NNZ_DENSE[method]
+    return 2 * (D - 1) * (nq + npv) + 2 * 3 * (nq + npv) + 24
 
 
 def f_save(method: str = "dopri8", f_rhs: int = F_RHS_MW) -> int:

@@ -63,11 +63,11 @@ def disk_ics(n: int, seed: int, R_range=(4.0, 20.0), z_sigma=0.3, dispersion=0.1
 
 
 def config_plain_small(seed: int = 1):
-    """1,024 MW orbits, Dopri8 fixed step 0.5 Myr over 0..1000 Myr.  Save
-    times lie on the step grid (every 20 steps: 0, 10, ..., 990 Myr) plus
-    the end point."""
+    """1,024 MW orbits, Dopri8 fixed step 0.5 Myr over 0..1000 Myr, 100 save
+    times linspace(0, 1000, 100): all but the two ends fall inside steps and
+    come from the method's dense output."""
     w0 = disk_ics(1024, seed)
-    ts = np.concatenate([np.arange(0.0, 1000.0, 10.0), [1000.0]])
+    ts = np.linspace(0.0, 1000.0, 100)
     return w0, 0.0, 1000.0, ts
 
 

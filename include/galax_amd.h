@@ -232,9 +232,12 @@ enum gx_ts_mode {
  * from t0[i] to t1[i] (t_stride 0: one shared pair t0[0], t1[0]), starting
  * from w0[n*6] = (x,y,z,vx,vy,vz), writing out[n*T*6] at the save times.
  * Each orbit takes its own step sequence (the batched-t0/t1 semantics of
- * orbit/solver.py:854-1039).  The state at an interior save time is the
- * method's own solution of a step from the last accepted step start to that
- * time.  Replaces the diffeqsolve call inside OrbitSolver.solve
+ * orbit/solver.py:854-1039).  The state at a save time inside an accepted
+ * step is the method's dense output over that step (diffrax's interpolant
+ * for SaveAt(ts), legacy/integrator.py:227-230).  With no dt0, a PID
+ * controller that sets dtmin starts from a first step of dtmin; otherwise
+ * the automatic first step is diffrax's.  Replaces the diffeqsolve call
+ * inside OrbitSolver.solve
  * (orbit/solver.py:774-1039) and thus the hot loop of gd.evaluate_orbit
  * (legacy/funcs.py:42-213).
  *
@@ -245,7 +248,7 @@ enum gx_ts_mode {
  * shortens the tail of a launch of uneven orbits.
  * `workspace` must point to at least gx_integrate_workspace_bytes(n) bytes of
  * device memory (256-byte aligned) that no concurrent call uses; it holds the
- * work-queue counter and each orbit's first step. */
+ * work-queue counter and each orbit's first step and start acceleration. */
 int64_t gx_integrate_workspace_bytes(int64_t n);
 int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
                  const double* w0, const double* t0, const double* t1,
@@ -253,6 +256,39 @@ int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
                  int32_t ts_mode, int64_t n, const int64_t* order,
                  double* out, int32_t* status, int64_t* nsteps,
                  int64_t* naccept, void* workspace, void* stream);
+
+/* Dense solutions (diffrax's dense interpolation, SaveAt(dense=True); the
+ * Orbit interpolant of legacy/funcs.py:210 and orbit/interp.py:42-145).
+ * gx_integrate_dense is gx_integrate that also writes, for every accepted
+ * step of orbit i, a record of GX_DENSE_RECORD doubles at
+ * rec[(i * rec_cap + k) * GX_DENSE_RECORD]: the step's start time t and
+ * signed step h (physical time), q and v there, and its stage accelerations
+ * (13 stages and the end stage for Dopri8, 7 for Dopri5).  nrec[i] receives
+ * the records written; an orbit whose accepted steps outnumber rec_cap stops
+ * with GX_MAX_STEPS_REACHED.  gx_dense_eval evaluates the method's dense
+ * output from such records at times t (per_orbit 0: t[T] for every orbit;
+ * 1: t[n*T]) into out[n*T*6]; a time outside an orbit's integrated interval
+ * gives NaN.  No separate reference entry point: diffrax keeps the same
+ * per-step data in its DenseInterpolation. */
+#define GX_DENSE_RECORD 50
+int gx_integrate_dense(const gx_pot* pot, const gx_solver_cfg* cfg,
+                       const double* w0, const double* t0, const double* t1,
+                       int32_t t_stride, const double* ts, int32_t T,
+                       int32_t ts_mode, int64_t n, const int64_t* order,
+                       double* out, int32_t* status, int64_t* nsteps,
+                       int64_t* naccept, double* rec, int64_t rec_cap,
+                       int64_t* nrec, void* workspace, void* stream);
+int gx_dense_eval(int32_t method, const double* rec, int64_t rec_cap,
+                  const int64_t* nrec, int64_t n, const double* t,
+                  int32_t per_orbit, int32_t T, double* out, void* stream);
+/* The same records from a shared-step solve (gx_integrate_shared below):
+ * every orbit's records share the batch's steps. */
+int gx_integrate_shared_dense(const gx_pot* pot, const gx_solver_cfg* cfg,
+                              const double* w0, double t0, double t1,
+                              const double* ts, int32_t T, int64_t n,
+                              double* out, int32_t* status, int64_t* nsteps,
+                              int64_t* naccept, double* rec, int64_t rec_cap,
+                              int64_t* nrec, void* workspace, void* stream);
 
 /* Integrate n orbits as ONE system with a shared step sequence: galax's
  * Integrator with scalar start and end times solves a batch (B, 6) as one
@@ -320,6 +356,17 @@ int gx_set_launch_shape(int32_t block, int32_t blocks_per_cu,
  * gx_set_launch_shape.  No reference counterpart (a tuning control). */
 int gx_set_launch_fraction(int32_t parts, int32_t* prev_parts);
 
+/* The persistent integrator's work-queue policy.  head_lanes (>= 0; < 0
+ * keeps the current value; default 4096): in a launch of at least four
+ * times as many lanes, the first head_lanes lanes start on the first
+ * head_lanes tickets (the longest orbits of a longest-first order), run at
+ * top wave priority and take no second ticket; 0 turns this off.  chunk
+ * (1..64; 0 keeps; default 1): tickets a lane takes per atomic.  Results do
+ * not depend on either.  No reference counterpart (a tuning control, set by
+ * galax_amd.tune). */
+int gx_set_queue_policy(int64_t head_lanes, int32_t chunk,
+                        int64_t* prev_head_lanes, int32_t* prev_chunk);
+
 /* Library and device information for tooling: number of compute units of
  * the current device, and the VGPR count of the integrator kernels. */
 int gx_device_info(int32_t* num_cu, int32_t* warp_size);
@@ -351,6 +398,18 @@ int gx_mlp_gemm(const void* A, int64_t sam, int64_t sak, const void* B,
                 int32_t epi, int32_t accumulate, int32_t ksplit, const float* bias,
                 float* C, int64_t ldc, void* H, int64_t ldh, const float* Zin,
                 int64_t ldzin, void* stream);
+/* The MLP's whole inference pass in one kernel, for hidden width 256:
+ * Y[m*ldy + o] = layer_depth(... silu(layer_0(X[m*ldx + 0..nin-1]))), with
+ * W[l] the BF16 weights [out][in] (W[0]: 256 x nin; W[1..depth-1]:
+ * 256 x 256; W[depth]: nout x 256; the last two 16-byte aligned) and b[l]
+ * the FP32 biases.  Activations stay on chip (rounded to BF16 between
+ * layers, as the layer-by-layer gx_mlp_gemm path rounds them).  W and b are
+ * host arrays of depth + 1 device pointers; nin <= 8, nout <= 16,
+ * depth <= 8. */
+int gx_mlp_forward_fused(const void* X, int32_t ldx, int32_t nin, int64_t n,
+                         int32_t depth, const void* const* W,
+                         const float* const* b, int32_t nout, float* Y,
+                         int32_t ldy, void* stream);
 /* out[n] += sum_m dz[m*ld + n] (bias gradients; out must hold the start). */
 int gx_mlp_colsum(const void* dz, int64_t ld, int64_t M, int64_t N, float* out,
                   void* stream);

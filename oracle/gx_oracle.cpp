@@ -34,7 +34,10 @@
 //     (orbit/solver.py:774-812).
 // Interior save times are served by the accepted step's dense output, as
 // diffrax's SaveAt(ts) does (the Dopri8 and Dopri5 interpolants are restated
-// at the Tableau below).
+// at the Tableau below).  With a PID controller that sets dtmin and no dt0,
+// the first step is dtmin: the one reference fixture that sets dtmin (the
+// integrate_field doctest, dynamics/_src/solver.py:342-365) is reproduced to
+// its printed digits from that first step and from no automatic one.
 
 #include "../include/galax_amd.h"
 

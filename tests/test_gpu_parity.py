@@ -27,21 +27,49 @@ def _rel(a, b, axis=-1):
     return np.linalg.norm(a - b, axis=axis) / np.maximum(np.linalg.norm(b, axis=axis), 1e-300)
 
 
-def check_adaptive(out, ref, nsteps_gpu, nsteps_ref, *, all_tol=1e-6, median_tol=1e-11, tight=1e-9, frac_tight=0.9):
-    """Adaptive parity, stated precisely.  Most orbits take the same step
-    sequence as the oracle and agree to round-off growth (the median is held
-    to `median_tol`, and at least `frac_tight` of the orbits to `tight`).  An
-    accept/reject decision taken on a scaled error within round-off of 1 can
-    flip between two correct implementations (the error estimate is a
-    heavily cancelling sum); such orbits then differ at the tolerance level,
-    amplified by the orbit's own sensitivity, bounded by `all_tol`."""
-    same = np.asarray(nsteps_gpu) == np.asarray(nsteps_ref)
-    e = np.maximum(_rel(out[..., :3], ref[..., :3]), _rel(out[..., 3:], ref[..., 3:])).max(axis=-1)
-    print(f"same step count {same.mean():.4f}; within {tight:.0e}: {np.mean(e < tight):.4f}; "
-          f"median {np.median(e):.2e}; max {e.max():.2e}", flush=True)
-    assert np.median(e) < median_tol
-    assert np.mean(e < tight) >= frac_tight
-    assert e.max() < all_tol
+def _orbit_err(out, ref):
+    """Per-orbit max over saves of the larger of |dq|/|q| and |dp|/|p|."""
+    e = np.maximum(_rel(out[..., :3], ref[..., :3]), _rel(out[..., 3:], ref[..., 3:]))
+    return e.reshape(e.shape[0], -1).max(axis=-1)
+
+
+def _stats(out, ref, ns, ns_ref, tight=1e-9):
+    e = _orbit_err(out, ref)
+    return {
+        "n": int(e.size),
+        "n_other_steps": int(np.sum(np.asarray(ns) != np.asarray(ns_ref))),
+        "n_loose": int(np.sum(~(e < tight))),
+        "median": float(np.median(e)),
+        "p99": float(np.quantile(e, 0.99)),
+        "max": float(e.max()),
+    }
+
+
+def noise_floor(pot, solver, w0, t0, t1, ref, ns_ref, **kw):
+    """How far the oracle moves from itself when every initial coordinate
+    moves by one ulp: an accept/reject decision taken on a scaled error
+    within round-off of 1 flips, and the orbit then separates at the
+    tolerance level, amplified by its own sensitivity.  No two correct FP64
+    implementations agree better than this."""
+    w1 = np.nextafter(np.asarray(w0, dtype=np.float64), np.inf)
+    ref1, st1, ns1, _ = oracle.integrate(pot, solver, w1, t0, t1, **kw)
+    return _stats(ref1, ref, ns1, ns_ref)
+
+
+def check_adaptive(out, ref, nsteps_gpu, nsteps_ref, floor):
+    """Adaptive parity against the measured noise floor (see noise_floor):
+    the GPU may take a different step count, or separate beyond 1e-9, on no
+    more orbits than the oracle does against its own 1-ulp-perturbed input
+    (plus a sampling allowance of three standard deviations of a Poisson
+    count, and two), its median error stays at round-off and its 99th
+    percentile within ten times the floor's."""
+    g = _stats(out, ref, nsteps_gpu, nsteps_ref)
+    print(f"gpu   {g}\nfloor {floor}", flush=True)
+    allow = lambda k: floor[k] + 3.0 * np.sqrt(floor[k] + 1.0) + 2.0
+    assert g["n_other_steps"] <= allow("n_other_steps")
+    assert g["n_loose"] <= allow("n_loose")
+    assert g["median"] <= max(10.0 * floor["median"], 1e-12)
+    assert g["p99"] <= max(10.0 * floor["p99"], 1e-9)
 
 
 # ------------------------------------------------------------------ fields
@@ -135,7 +163,8 @@ def test_fixed_step_parity_more_models(name):
 
 
 def test_fixed_step_plain_small_parity():
-    """1,024 MW orbits, Dopri8 fixed step 0.5 Myr, 0..1 Gyr, on-grid saves."""
+    """1,024 MW orbits, Dopri8 fixed step 0.5 Myr, 0..1 Gyr, 100 saves (98
+    of them inside steps: the dense output)."""
     w0, t0, t1, ts = workloads.config_plain_small()
     solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.ConstantStepSize(), max_steps=None)
     say("fixed-step plain small", w0.shape, len(ts))
@@ -181,16 +210,20 @@ def test_fixed_step_specialised_evaluators(name):
 
 
 def test_adaptive_uneven_parity():
-    """Per-orbit end times, rtol 1e-8 / atol 1e-10, per-orbit linspace saves."""
-    w0, t0, T = workloads.config_uneven(n=512, seed=5)
-    T = np.minimum(T, 600.0)
+    """The throughput configuration at full horizon on a 2,048-orbit sample:
+    R ~ U(0.5, 50) kpc, 25 % dispersion, T_i log-uniform in 0.1-5 Gyr, adaptive
+    Dopri8 rtol 1e-8 / atol 1e-10, 64 per-orbit linspace saves (62 of them
+    from the dense output), against the oracle's noise floor on the same
+    sample."""
+    w0, t0, T = workloads.config_uneven(n=2048, seed=5)
     solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.PIDController(rtol=1e-8, atol=1e-10), max_steps=None)
-    say("adaptive uneven", w0.shape)
-    res = gd.integrate_orbits(gp.MilkyWayPotential(), w0, t0, T, T=16, solver=solver)
+    say("adaptive uneven full horizon", w0.shape, float(T.max()))
+    res = gd.integrate_orbits(gp.MilkyWayPotential(), w0, t0, T, T=64, solver=solver)
     out = res.out.cpu().numpy()
-    ref, st, ns, na = oracle.integrate(gp.MilkyWayPotential(), solver, w0, t0, T, T=16)
-    assert np.all(res.status.cpu().numpy() == 0)
-    check_adaptive(out, ref, res.nsteps.cpu().numpy(), ns, all_tol=1e-6)
+    ref, st, ns, na = oracle.integrate(gp.MilkyWayPotential(), solver, w0, t0, T, T=64)
+    assert np.all(res.status.cpu().numpy() == 0) and np.all(st == 0)
+    floor = noise_floor(gp.MilkyWayPotential(), solver, w0, t0, T, ref, ns, T=64)
+    check_adaptive(out, ref, res.nsteps.cpu().numpy(), ns, floor)
 
 
 @pytest.mark.parametrize("key", ["solve_single_pid1e8", "solve_single_raw_array_pid1e8", "solve_batched_t1_independent_pid1e8"])
@@ -203,6 +236,76 @@ def test_adaptive_doctests_on_gpu(golden, key):
     out = res.out.cpu().numpy()
     np.testing.assert_allclose(out[:, 0, :3], np.array(e["q"]).reshape(-1, 3), atol=5.5e-9)
     np.testing.assert_allclose(out[:, 0, 3:], np.array(e["p"]).reshape(-1, 3), atol=5.5e-9)
+
+
+def test_kepler_integrate_field_doctest_on_gpu(golden):
+    """galax's default integrate_field solver (Dopri8, PID 1e-7, dtmin 0.05
+    forced; dynamics/_src/solver.py:226-232,342-365) on the GPU: ys[1] and
+    ys[-2] lie inside steps (dense output), ys[-1] is the end point; all
+    three to the doctest's eight printed decimals."""
+    e = golden["integrator"]["integrate_field_kepler_pid1e7"]
+    pot = from_spec(e["potential"]["class"], e["potential"]["params"])
+    w0, _ = ic(e)
+    a, b, n = e["ts_linspace"]
+    ts = np.linspace(a, b, n)
+    say("kepler integrate_field doctest")
+    solver = solver_from(e["solver"])
+    for shared in (False, True):
+        res = gd.integrate_orbits(pot, w0, 0.0, 1000.0, ts=ts, solver=solver, shared=shared)
+        out = res.out.cpu().numpy()
+        assert np.all(res.status.cpu().numpy() == 0)
+        for k in ("1", "-2", "-1"):
+            np.testing.assert_allclose(out[0, int(k), :3], e["ys_q"][k], atol=5.5e-9)
+            np.testing.assert_allclose(out[0, int(k), 3:], e["ys_p"][k], atol=5.5e-9)
+
+
+def test_dense_output_parity_fixed_step():
+    """Off-grid saves from the dense output agree with the oracle's
+    interpolant to 1e-12 (Dopri8 and Dopri5, forward and backward)."""
+    pot = gp.MilkyWayPotential()
+    w0 = workloads.disk_ics(256, 43)
+    rng = np.random.default_rng(7)
+    say("dense output fixed step")
+    for meth in (gd.Dopri8(), gd.Dopri5()):
+        solver = gd.OrbitSolver(meth, stepsize_controller=gd.ConstantStepSize(), max_steps=None)
+        for t1 in (200.0, -200.0):
+            # ordered in the direction of integration
+            ts = np.sort(rng.uniform(0.0, abs(t1), 37)) * np.sign(t1)
+            res = gd.integrate_orbits(pot, w0, 0.0, t1, ts=ts, solver=solver, dt0=1.3)
+            ref, *_ = oracle.integrate(pot, solver, w0, 0.0, t1, ts=ts, dt0=1.3)
+            out = res.out.cpu().numpy()
+            assert np.max(_rel(out[..., :3], ref[..., :3])) < 1e-12
+            assert np.max(_rel(out[..., 3:], ref[..., 3:])) < 1e-12
+
+
+@pytest.mark.parametrize("fixed", [True, False])
+def test_dense_orbit_interpolant_matches_oracle(fixed):
+    """evaluate_orbit(dense=True): the interpolant evaluates the solve's own
+    step records on the device at any time.  Against the oracle's saves at
+    the same times from the same solve (a batch with scalar times: one
+    shared step sequence, as galax's Integrator solves it)."""
+    pot = gp.MilkyWayPotential()
+    w0 = workloads.disk_ics(64, 47)
+    if fixed:
+        integ = gd.Integrator(gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.ConstantStepSize(),
+                                             max_steps=None), diffeq_kw={"dt0": 0.7})
+    else:
+        integ = gd.Integrator()
+    t = np.linspace(0.0, 500.0, 5)
+    say("dense orbit interpolant", "fixed" if fixed else "pid 1e-7 shared")
+    orb = gd.evaluate_orbit(pot, w0, t, integrator=integ, dense=True)
+    tq = np.sort(np.random.default_rng(5).uniform(0.0, 500.0, 41))
+    got = orb(tq)
+    got = np.concatenate([np.asarray(got.q), np.asarray(got.p)], -1)
+    ref, st, ns, _ = oracle.integrate(pot, integ.dynamics_solver, w0, 0.0, 500.0, ts=tq,
+                                      dt0=integ.diffeq_kw.get("dt0"), shared=not fixed)
+    assert int(np.max(orb.nsteps)) == int(ns[0])
+    tol = 1e-12 if fixed else 1e-9
+    assert np.max(_rel(got[..., :3], ref[..., :3])) < tol
+    assert np.max(_rel(got[..., 3:], ref[..., 3:])) < tol
+    # the interpolant reproduces the solve's own saves
+    o2 = orb(t)
+    np.testing.assert_allclose(np.asarray(o2.q), np.asarray(orb.q), rtol=0, atol=1e-11)
 
 
 def test_one_step_doctest_on_gpu(golden):
@@ -229,10 +332,8 @@ def test_adaptive_parity_moving_potentials(name):
     ref, st, ns, na = oracle.integrate(pot, solver, w0, -40.0, 300.0, T=8)
     out = res.out.cpu().numpy()
     assert np.all(res.status.cpu().numpy() == 0)
-    # A moving centre (a track interpolated per stage, a rotating frame) adds
-    # round-off to every field value, so more accept/reject decisions flip than
-    # for a static field; the median still pins agreement at round-off level.
-    check_adaptive(out, ref, res.nsteps.cpu().numpy(), ns, all_tol=1e-5, frac_tight=0.7)
+    floor = noise_floor(pot, solver, w0, -40.0, 300.0, ref, ns, T=8)
+    check_adaptive(out, ref, res.nsteps.cpu().numpy(), ns, floor)
 
 
 def test_dopri5_parity():
@@ -245,7 +346,8 @@ def test_dopri5_parity():
         if isinstance(ctl, gd.ConstantStepSize):
             assert np.max(_rel(res.out.cpu().numpy()[..., :3], ref[..., :3])) < 1e-12
         else:
-            check_adaptive(res.out.cpu().numpy(), ref, res.nsteps.cpu().numpy(), ns, all_tol=1e-6)
+            floor = noise_floor(gp.MilkyWayPotential(), solver, w0, 0.0, 250.0, ref, ns, T=5)
+            check_adaptive(res.out.cpu().numpy(), ref, res.nsteps.cpu().numpy(), ns, floor)
 
 
 def test_save_modes_and_backward():
@@ -314,7 +416,8 @@ def test_queue_head_policy_does_not_change_results():
     np.testing.assert_array_equal(a.nsteps.cpu().numpy(), b.nsteps.cpu().numpy())
     idx = np.random.default_rng(4).choice(n, 96, replace=False)
     ref, st, ns, na = oracle.integrate(gp.MilkyWayPotential(), solver, w0[idx], t0[idx], T[idx], T=3)
-    check_adaptive(oa[idx], ref, a.nsteps.cpu().numpy()[idx], ns, all_tol=1e-6)
+    floor = noise_floor(gp.MilkyWayPotential(), solver, w0[idx], t0[idx], T[idx], ref, ns, T=3)
+    check_adaptive(oa[idx], ref, a.nsteps.cpu().numpy()[idx], ns, floor)
 
 
 def test_batches_in_flight_do_not_change_results():

@@ -168,7 +168,7 @@ def test_solver_configuration():
 
 def test_workloads_are_seeded_and_shaped():
     w0, t0, t1, ts = workloads.config_plain_small()
-    assert w0.shape == (1024, 6) and ts[0] == 0.0 and ts[-1] == 1000.0 and len(ts) == 101
+    assert w0.shape == (1024, 6) and ts[0] == 0.0 and ts[-1] == 1000.0 and len(ts) == 100
     np.testing.assert_array_equal(w0, workloads.config_plain_small()[0])
     R = np.hypot(w0[:, 0], w0[:, 1])
     assert R.min() >= 4.0 and R.max() <= 20.0
@@ -233,3 +233,44 @@ def test_integrator_kernels_keep_two_waves_per_simd(tmp_path):
             assert vgpr <= 256, (n, vgpr)
             if max_spill is not None:
                 assert spill <= max_spill, (n, spill)
+
+
+def test_caller_buffers_are_checked():
+    """integrate_orbits rejects an output or workspace buffer that the
+    kernels would overrun or misread (checked before any launch)."""
+    import torch
+
+    from galax_amd.dynamics._solver import _check_out, _check_workspace
+
+    with pytest.raises(ValueError, match="shape"):
+        _check_out(torch, torch.empty((4, 3, 6), dtype=torch.float64), 5, 3)
+    with pytest.raises(ValueError, match="float64 CUDA"):
+        _check_out(torch, torch.empty((5, 3, 6), dtype=torch.float64), 5, 3)
+    with pytest.raises(ValueError, match="float64 CUDA"):
+        _check_out(torch, torch.empty((5, 3, 6), dtype=torch.float32), 5, 3)
+    with pytest.raises(ValueError, match="bytes"):
+        _check_workspace(torch, torch.empty(10, dtype=torch.float64), 81)
+    with pytest.raises(ValueError, match="CUDA"):
+        _check_workspace(torch, torch.empty(11, dtype=torch.float64), 81)
+    with pytest.raises(ValueError):
+        _check_out(torch, np.zeros((5, 3, 6)), 5, 3)
+
+
+def test_autotuner_search_finds_the_minimum_and_measures_each_setting_once():
+    from galax_amd.tune import SPACE, START, coordinate_descent
+
+    calls = []
+
+    def cost(c):
+        calls.append(tuple(sorted(c.items())))
+        # a separable bowl with its minimum off the start point
+        return (1.0 + abs(c["in_flight"] - 4) + 0.5 * abs(c["parts"] - 3) + (c["blocks_per_cu"] != 2)
+                + 0.1 * (c["head_lanes"] != 1024) + 0.2 * (c["chunk"] != 1))
+
+    best, v, trials = coordinate_descent(cost, SPACE, START, margin=0.02)
+    assert best == {"in_flight": 4, "parts": 3, "blocks_per_cu": 2, "head_lanes": 1024, "chunk": 1}
+    assert v == pytest.approx(1.0)
+    assert len(calls) == len(set(calls)) == len(trials)
+    # a change within the margin is not taken
+    best2, _, _ = coordinate_descent(lambda c: 1.0 - 0.001 * (c["chunk"] == 4), SPACE, START, margin=0.02)
+    assert best2 == START

@@ -87,6 +87,8 @@ def test_stream_release_integration_parity():
     ref, st, _ns, _ = oracle.integrate(field, gen.stream_integrator.dynamics_solver, w_all,
                                    np.concatenate([ts, ts]), ts[-1] + 1e-3, max_steps=None)
     got = np.concatenate([stream["lead"].w(), stream["trail"].w()])
-    from test_gpu_parity import check_adaptive
+    from test_gpu_parity import check_adaptive, noise_floor
 
-    check_adaptive(got[:, None, :], ref, stream.nsteps, _ns, all_tol=1e-5, frac_tight=0.8)
+    floor = noise_floor(field, gen.stream_integrator.dynamics_solver, w_all, np.concatenate([ts, ts]),
+                        ts[-1] + 1e-3, ref, _ns, max_steps=None)
+    check_adaptive(got[:, None, :], ref, stream.nsteps, _ns, floor)

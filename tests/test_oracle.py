@@ -578,3 +578,28 @@ def test_powerlaw_cutoff_against_scipy_incomplete_gamma():
         dphi = GM * gammainc(a1, s2) / r**2
         np.testing.assert_allclose(oracle.evaluate(pot, "potential", x), phi, rtol=1e-12)
         np.testing.assert_allclose(oracle.evaluate(pot, "gradient", x)[:, 0], dphi, rtol=1e-11)
+
+
+@pytest.mark.parametrize("method,name", [(_abi.GX_DOPRI8, "dopri8"), (_abi.GX_DOPRI5, "dopri5")])
+def test_flop_model_counts_match_the_tableaux(method, name):
+    """The roofline's work counts (galax_amd/flops.py) read the tableau's and
+    the dense output's nonzero structure: check them against the oracle's."""
+    from galax_amd import flops
+
+    c, A, b, be, order = oracle.tableau(method)
+    s = len(c)
+    assert flops.NNZ[name] == (s, int(np.sum(A != 0)), int(np.sum(b != 0)), int(np.sum(be != 0)))
+    ths = np.linspace(0.1, 0.9, 9)
+    W = np.array([oracle.dense_weights(method, t) for t in ths])
+    sf = W.shape[1]
+    Ae = np.zeros((sf, sf))
+    Ae[:s, :s] = A
+    if sf > s:
+        Ae[s, :s] = b
+    D, nq, npv = flops.NNZ_DENSE[name]
+    assert nq == int(np.sum(np.abs(W @ Ae).max(0) > 1e-14))
+    assert npv == int(np.sum(np.abs(W).max(0) > 1e-14))
+    # the weights are th times a polynomial of degree D - 1
+    for j in range(sf):
+        coef = np.polyfit(ths, W[:, j] / ths, D - 1)
+        np.testing.assert_allclose(np.polyval(coef, ths), W[:, j] / ths, atol=1e-9)

@@ -46,8 +46,8 @@ def test_shared_entry_point_exported():
         assert hasattr(L, name), name
     ws = _lib.lib().gx_integrate_shared_workspace_bytes
     # control record + ticket + two partials per 256-orbit workgroup + two
-    # state buffers and the start accelerations
-    assert ws(1000) == 512 + 16 * 4 + 8 * 15 * 1000
+    # state buffers of (q, p, step-start acceleration)
+    assert ws(1000) == 512 + 16 * 4 + 8 * 18 * 1000
     assert ws(-1) == 0
 
 

@@ -114,6 +114,32 @@ def test_gemm_scalar_strides_and_bad_arguments():
         S.gemm(A, 66, 2, W, 20, 1, 50, 20, 33, _abi.GX_EPI_BIAS_SILU, bias=C[0], ldc=20, ksplit=2)
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 1000])
+def test_fused_inference_matches_layer_by_layer(n):
+    """gx_mlp_forward_fused (one kernel, activations in LDS) against the
+    layer-by-layer GEMM path on the same BF16 weights: both accumulate in
+    FP32 and round activations to BF16, so they differ only where an
+    accumulation order moves a BF16 rounding."""
+    import galax_amd.surrogate as S
+
+    torch = _torch()
+    mlp = S.MLP(hidden=256, depth=4, seed=3)
+    gen = torch.Generator(device="cuda").manual_seed(1)
+    for w in mlp.b:
+        w.copy_(torch.randn(w.shape, generator=gen, device="cuda") * 0.1)
+    X = torch.zeros((n, S.K_IN), dtype=torch.bfloat16, device="cuda")
+    X[:, :5] = torch.randn((n, 5), generator=gen, device="cuda").to(torch.bfloat16)
+    Y0 = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    Y1 = torch.full((n, 3), float("nan"), dtype=torch.float32, device="cuda")
+    mlp.forward(X, n, Y0)
+    mlp.forward_fused(X, n, Y1)
+    d = (Y1 - Y0).abs().max().item()
+    scale = Y0.abs().max().item()
+    print(f"n={n} max |fused - layered| {d:.3e} of {scale:.3e}", flush=True)
+    assert torch.isfinite(Y1).all()
+    assert d <= 2e-2 * scale
+
+
 def test_adam_matches_formula():
     import galax_amd.surrogate as S
     from galax_amd import _lib
@@ -258,3 +284,25 @@ def test_surrogate_learns_the_perturbation():
     e_mw = ((mwa - exact).norm(dim=1) / exact.norm(dim=1)).median().item()
     print(f"median rel error: surrogate {e_sur:.3e}, ignoring perturbers {e_mw:.3e}", flush=True)
     assert e_sur < 0.8 * e_mw
+
+
+def test_surrogate_configuration_against_the_adaptive_reference():
+    """The learned-surrogate configuration at reduced size (16,384 training
+    orbits, 2,048 new orbits, 1 Gyr, gate on): trained net, lock-step
+    integration, the exact field at the same fixed step, and the reference
+    solution --
+    the exact field with adaptive Dopri8 at rtol 1e-8, whose Jacobi energy
+    holds to round-off.  Reports median and max position error and Jacobi
+    drift; the trained network must beat the Milky Way model alone (the
+    method without it) in median position error after 1 Gyr."""
+    import galax_amd.surrogate as S
+
+    print("[case] surrogate configuration vs adaptive reference", flush=True)
+    rep = S.barred_halo_experiment(n_train=16384, n_bench=2048, t1=1000.0, epochs=20)
+    e = rep["pos_rel_err_vs_reference"]
+    print({k: rep[k] for k in ("reference_ms", "exact_fixed_ms", "surrogate_ms", "surrogate_speedup_vs_exact_fixed_step",
+                               "flagged_orbits")}, flush=True)
+    print("position error", e, "\njacobi drift", rep["jacobi_drift_median_max"], flush=True)
+    assert np.isfinite(e["surrogate"]["max"]) and np.isfinite(e["exact_fixed_step"]["max"])
+    assert rep["jacobi_drift_median_max"]["reference"][0] < 1e-9
+    assert e["surrogate"]["median"] < e["milky_way_only"]["median"]
