@@ -115,6 +115,14 @@ def lib() -> ctypes.CDLL:
                 f"{LIB_PATH} is missing: build it with `python -m galax_amd._build` "
                 "(or __graft_entry__.build())"
             )
+        # torch ships its own libamdhip64 under the same soname as the system
+        # ROCm's.  Loading this library first would map the system runtime
+        # and torch would then map its own beside it: two HIP runtimes in one
+        # process, whose device memory and streams are not each other's
+        # (calls fail with runtime errors).  Importing torch first makes the
+        # library bind to the runtime torch already holds.
+        import torch  # noqa: F401
+
         handle = ctypes.CDLL(str(LIB_PATH))
         _declare(handle)  # every signature, or none: never a half-declared handle
         _lib = handle

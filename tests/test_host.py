@@ -35,6 +35,24 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_library_shares_torchs_hip_runtime():
+    """Loading the library before anything imported torch must still leave
+    one HIP runtime in the process: torch bundles its own libamdhip64 under
+    the system runtime's soname, and two runtimes side by side do not share
+    device memory or streams."""
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libgalaxhip.so not built")
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from galax_amd import _lib; _lib.lib()\n"
+        "print(sorted({l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}))\n" % str(ROOT)
+    )
+    res = subprocess.run(["python3", "-c", code], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr[-2000:]
+    mapped = eval(res.stdout.strip().splitlines()[-1])
+    assert len(mapped) == 1, mapped
+
+
 def test_stale_library_is_refused_whole(tmp_path, monkeypatch):
     """A library that lacks an entry point of this package is refused before
     any signature is declared, and the loader keeps no handle: a later call
