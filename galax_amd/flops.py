@@ -66,4 +66,4 @@ PEAK_BF16_TFLOPS = 2500.0
 # copy: x2.0 and x1.0), for bench.py's workloads at 1,048,576 orbits and 64
 # saves.  The throughput workload's measurement and commands are kept in
 # profiles/k_integrate_mw_uneven_traffic.txt.
-MEASURED_TRAFFIC_BYTES = {"uneven": 5.4314e9}
+MEASURED_TRAFFIC_BYTES = {"uneven": 3.8973e9}
