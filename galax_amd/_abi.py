@@ -6,7 +6,7 @@ import ctypes
 
 c_double_p = ctypes.c_void_p
 
-GX_OK, GX_EINVAL, GX_ELAUNCH, GX_ENOMEM = 0, -1, -2, -3
+GX_OK, GX_EINVAL, GX_ELAUNCH, GX_ENOMEM, GX_ETOOBIG = 0, -1, -2, -3, -4
 
 GX_NULL = 0
 GX_KEPLER = 1

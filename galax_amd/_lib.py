@@ -42,6 +42,11 @@ SIGNATURES = {
         ctypes.c_int,
         [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     ),
+    "gx_integrate_noalloc_max_orbits": (_i64, []),
+    "gx_integrate_noalloc": (
+        ctypes.c_int,
+        [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    ),
     "gx_integrate_dense": (
         ctypes.c_int,
         [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
@@ -131,7 +136,8 @@ def lib() -> ctypes.CDLL:
 
 def check(rc: int, what: str) -> None:
     if rc != _abi.GX_OK:
-        names = {-1: "invalid argument", -2: "HIP launch/runtime error", -3: "out of device memory"}
+        names = {-1: "invalid argument", -2: "HIP launch/runtime error", -3: "out of device memory",
+                 -4: "more than the fixed-capacity entry point holds"}
         raise GalaxHipError(f"{what} failed: {names.get(rc, rc)} ({rc})")
 
 

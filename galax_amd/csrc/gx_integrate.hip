@@ -30,6 +30,7 @@
 // order, RMS norm, factormin applied to rejected steps only), its automatic
 // initial step and its clip-to-end rule.
 #include <cmath>
+#include <mutex>
 
 #include "gx_internal.hpp"
 #include "gx_rk.hpp"
@@ -639,6 +640,36 @@ int setup_args(const gx_pot* pot, const gx_solver_cfg* cfg, const double* w0,
   return GX_OK;
 }
 
+// The allocation-free entry point's workspace: a fixed array in device
+// memory, one copy per GPU for the life of the process.  Host code never
+// uses the array's name as an address: each device's copy is looked up once
+// with hipGetSymbolAddress (with that device current) and that pointer is
+// what kernels receive.  Nothing in it needs clearing between calls: every
+// gx_integrate launch zeroes the queue counter itself (k_reset_queue) and
+// the first-step prepass writes each orbit's h0 and a0 before the
+// integrator reads them.
+constexpr int64_t kScratchBytes = (int64_t)4 << 20;  // 4 MiB: 131,064 orbits
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_for_current_device(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return GX_ELAUNCH;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+      return GX_ELAUNCH;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return GX_OK;
+}
+
 }  // namespace
 }  // namespace gx
 
@@ -748,6 +779,27 @@ int gx_dense_eval(int32_t method, const double* rec, int64_t rec_cap,
   else
     return GX_EINVAL;
   return check(hipGetLastError());
+}
+
+int64_t gx_integrate_noalloc_max_orbits(void) {
+  return (kScratchBytes - 256) / 32;
+}
+
+int gx_integrate_noalloc(const gx_pot* pot, const gx_solver_cfg* cfg,
+                         const double* w0, const double* t0, const double* t1,
+                         int32_t t_stride, const double* ts, int32_t T,
+                         int32_t ts_mode, int64_t n, const int64_t* order,
+                         double* out, int32_t* status, int64_t* nsteps,
+                         int64_t* naccept, void* stream) {
+  if (n > gx_integrate_noalloc_max_orbits()) return GX_ETOOBIG;
+  if (n <= 0)  // nothing to launch; gx_integrate's argument checks
+    return gx_integrate(pot, cfg, w0, t0, t1, t_stride, ts, T, ts_mode, n,
+                        order, out, status, nsteps, naccept, nullptr, stream);
+  void* ws = nullptr;
+  const int rc = scratch_for_current_device(&ws);
+  if (rc != GX_OK) return rc;
+  return gx_integrate(pot, cfg, w0, t0, t1, t_stride, ts, T, ts_mode, n, order,
+                      out, status, nsteps, naccept, ws, stream);
 }
 
 }  // extern "C"

@@ -33,6 +33,7 @@ extern "C" {
 #define GX_EINVAL (-1)   /* bad argument (null pointer, bad size, bad enum) */
 #define GX_ELAUNCH (-2)  /* HIP launch / runtime error */
 #define GX_ENOMEM (-3)   /* device allocation failed (create calls only) */
+#define GX_ETOOBIG (-4)  /* more than a fixed-capacity entry point holds */
 
 /* ------------------------------------------------------------ components */
 /* Potential models (potential/_src/builtin/...).  Parameter block p[]:
@@ -256,6 +257,23 @@ int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
                  int32_t ts_mode, int64_t n, const int64_t* order,
                  double* out, int32_t* status, int64_t* nsteps,
                  int64_t* naccept, void* workspace, void* stream);
+
+/* gx_integrate without a workspace argument and without any allocation, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  No counterpart in the reference.  The workspace is
+ * a fixed 4 MiB array in device memory, one per GPU, held until the process
+ * ends, so n is at most gx_integrate_noalloc_max_orbits() (131,064); larger
+ * n returns GX_ETOOBIG.  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams, other host threads) race, so a
+ * caller serialises them.  The first call on a device looks the array up
+ * (hipGetSymbolAddress); no call allocates.  Otherwise as gx_integrate. */
+int64_t gx_integrate_noalloc_max_orbits(void);
+int gx_integrate_noalloc(const gx_pot* pot, const gx_solver_cfg* cfg,
+                         const double* w0, const double* t0, const double* t1,
+                         int32_t t_stride, const double* ts, int32_t T,
+                         int32_t ts_mode, int64_t n, const int64_t* order,
+                         double* out, int32_t* status, int64_t* nsteps,
+                         int64_t* naccept, void* stream);
 
 /* Dense solutions (diffrax's dense interpolation, SaveAt(dense=True); the
  * Orbit interpolant of legacy/funcs.py:210 and orbit/interp.py:42-145).
