@@ -120,11 +120,17 @@ def stream_report(steps: int) -> dict:
     back = gd.evaluate_orbit(mw, w0, np.array([0.0, ts[0]]), integrator=integ)
     w_start = gd.PhaseSpaceCoordinate(q=np.asarray(back.q).reshape(-1, 3)[-1], p=np.asarray(back.p).reshape(-1, 3)[-1],
                                       t=ts[0])
-    torch.cuda.synchronize()
-    t_run = time.perf_counter()
-    stream, _ = gen.run(3, ts, w_start, 1e5)
-    torch.cuda.synchronize()
-    ms_run = (time.perf_counter() - t_run) * 1e3
+    # MockStreamGenerator.run end to end (progenitor orbit, release sampling,
+    # stream field, particle integration, read-back): the first call in the
+    # process also loads the kernels' code objects; later calls are what a
+    # long-running caller sees
+    runs = []
+    for _ in range(1 + max(steps, 1)):
+        torch.cuda.synchronize()
+        t_run = time.perf_counter()
+        stream, _ = gen.run(3, ts, w_start, 1e5)
+        torch.cuda.synchronize()
+        runs.append((time.perf_counter() - t_run) * 1e3)
     # the release integration alone, on device-resident ICs
     prog_o = gd.evaluate_orbit(mw, w_start, ts, integrator=integ)
     prog_o.q = np.asarray(prog_o.q).reshape(-1, 3)
@@ -155,7 +161,9 @@ def stream_report(steps: int) -> dict:
         "max_steps_one_particle": int(nst.max()),
         "ms_release_integration": ms,
         "orbit_steps_per_s": float(nst.sum()) / (ms * 1e-3),
-        "ms_generator_run_end_to_end": ms_run,
+        "ms_generator_run_end_to_end": float(np.mean(runs[1:])),
+        "ms_generator_run_end_to_end_runs": [round(r, 2) for r in runs[1:]],
+        "ms_generator_run_first_call": runs[0],
         "stream_failed_in_run": int((stream.status != 0).sum()),
     }
 

@@ -34,6 +34,16 @@ HIPCC_FLAGS = [
 ]
 
 
+# Per-source additions.  The persistent integrator's loop holds more
+# loop-invariant values than the scalar register file: with machine LICM the
+# compiler hoists constants and address arithmetic out of the loop and then
+# spills them to vector lanes, reloading them on every trip.  Without it the
+# Milky Way kernel needs 181 VGPRs instead of 250 and half the lane reads,
+# and the throughput pass measured 19.1-19.4 ms instead of 20.1-20.5 ms
+# (alternating runs on two boxes, same step counts).
+SOURCE_FLAGS = {"gx_integrate.hip": ["-mllvm", "-disable-machine-licm"]}
+
+
 def hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and Path(cand).exists():
@@ -56,7 +66,8 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     jobs = jobs or max(1, min(len(srcs), len(os.sched_getaffinity(0))))
     with tempfile.TemporaryDirectory(prefix="galax_amd_build_") as tmpdir:
         objs = [str(Path(tmpdir) / (s.stem + ".o")) for s in srcs]
-        cmds = [[cc, *HIPCC_FLAGS, "-c", str(s), "-o", o] for s, o in zip(srcs, objs)]
+        cmds = [[cc, *HIPCC_FLAGS, *SOURCE_FLAGS.get(s.name, []), "-c", str(s), "-o", o]
+                for s, o in zip(srcs, objs)]
         with ThreadPoolExecutor(max_workers=jobs) as ex:
             for f in [ex.submit(_run, c, verbose) for c in cmds]:
                 f.result()

@@ -75,6 +75,24 @@ struct DPot {
 };
 
 // ------------------------------------------------------------- tracks
+// Track records live in device global memory; loads through a pointer
+// marked so compile to global (not flat) loads, which the compiler can wait
+// on one at a time instead of draining every outstanding memory and LDS
+// operation.
+typedef const __attribute__((address_space(1))) double GlobalDouble;
+
+// Record k as 8 pairs: t_k, t_k+1 | 1/(t_k+1 - t_k), unused | then the x, y
+// and z cubics' power-basis coefficients, two pairs each.
+__device__ __forceinline__ void load_track_record(const double* rec,
+                                                  double2 (&v)[8]) {
+  GlobalDouble* r = (GlobalDouble*)rec;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    v[j].x = r[2 * j];
+    v[j].y = r[2 * j + 1];
+  }
+}
+
 __device__ inline void track_pos(const DTrack& T, double t, double& x,
                                  double& y, double& z) {
   if (T.n <= 1 || t <= T.t0) {
@@ -98,10 +116,8 @@ __device__ inline void track_pos(const DTrack& T, double t, double& x,
   const long long last = T.n - 2;
   long long k = (long long)floor((t - T.t0) * T.inv_h);
   k = k < 0 ? 0 : (k > last ? last : k);
-  const double2* r2 = reinterpret_cast<const double2*>(T.rec + 16 * k);
   double2 v[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) v[i] = r2[i];
+  load_track_record(T.rec + 16 * k, v);
   if (!((k == 0 || v[0].x <= t) && (k == last || v[0].y > t))) {
     const double* r = T.rec + 16 * k;
     while (k > 0 && r[0] > t) {
@@ -112,15 +128,57 @@ __device__ inline void track_pos(const DTrack& T, double t, double& x,
       ++k;
       r += 16;
     }
-    r2 = reinterpret_cast<const double2*>(r);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] = r2[i];
+    load_track_record(r, v);
   }
   // record: t_k, t_k+1, 1/(t_k+1 - t_k), unused, then the x, y, z cubics
   const double s = (t - v[0].x) * v[1].x;
   x = fma(s, fma(s, fma(s, v[3].y, v[3].x), v[2].y), v[2].x);
   y = fma(s, fma(s, fma(s, v[5].y, v[5].x), v[4].y), v[4].x);
   z = fma(s, fma(s, fma(s, v[7].y, v[7].x), v[6].y), v[6].x);
+}
+
+// Track positions at N times, the same values as track_pos gives at each
+// time, with the N records' loads issued together: one memory round trip
+// for the N lookups instead of N in turn.  Every record is loaded and its
+// cubic evaluated unconditionally (a time outside the track reads a clamped
+// record whose value is then replaced; a one-knot track has one placeholder
+// record), so that no load waits behind another lookup's branch; only a
+// wrong guess (t within rounding of a knot) walks and reloads.
+template <int N>
+__device__ __forceinline__ void track_pos_n(const DTrack& T,
+                                            const double (&t)[N],
+                                            double (&d)[N][3]) {
+  const long long last = T.n - 2;
+  const double ulast = last > 0 ? (double)last : 0.0;
+  long long k[N];
+  double2 v[N][8];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    k[i] = (long long)fmin(fmax((t[i] - T.t0) * T.inv_h, 0.0), ulast);
+#pragma unroll
+  for (int i = 0; i < N; ++i) load_track_record(T.rec + 16 * k[i], v[i]);
+  bool fix = false;
+  bool bad[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    // record: t_k, t_k+1, 1/(t_k+1 - t_k), unused, then the x, y, z cubics
+    const double s = (t[i] - v[i][0].x) * v[i][1].x;
+    d[i][0] = fma(s, fma(s, fma(s, v[i][3].y, v[i][3].x), v[i][2].y), v[i][2].x);
+    d[i][1] = fma(s, fma(s, fma(s, v[i][5].y, v[i][5].x), v[i][4].y), v[i][4].x);
+    d[i][2] = fma(s, fma(s, fma(s, v[i][7].y, v[i][7].x), v[i][6].y), v[i][6].x);
+    const bool lo = T.n <= 1 || t[i] <= T.t0, hi = !lo && t[i] >= T.t1;
+    d[i][0] = lo ? T.q0[0] : (hi ? T.q1[0] : d[i][0]);
+    d[i][1] = lo ? T.q0[1] : (hi ? T.q1[1] : d[i][1]);
+    d[i][2] = lo ? T.q0[2] : (hi ? T.q1[2] : d[i][2]);
+    bad[i] = !lo && !hi &&
+             !((k[i] == 0 || v[i][0].x <= t[i]) && (k[i] == last || v[i][0].y > t[i]));
+    fix = fix || bad[i];
+  }
+  if (fix) {
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      if (bad[i]) track_pos(T, t[i], d[i][0], d[i][1], d[i][2]);
+  }
 }
 
 __device__ inline void frame_delta(const DPot& P, const DComp& c, double t,
@@ -842,24 +900,46 @@ struct EvalMilkyWayRotating {
 // The mock-stream field: the Milky Way model (rows 0-3) plus a Plummer
 // sphere riding a track (row 4), as MockStreamGenerator builds it with a
 // progenitor potential (legacy/mockstream/mockstream_generator.py:161-275).
+//
+// A step's stage times are known when the step starts, so the integrator
+// looks up the track at all of them up front (track_n: two batches of loads
+// instead of one dependent round trip per stage) and evaluates each stage
+// with its position (acc_at).  The prefetched positions take the registers
+// that the stage accelerations fill later, and the batched loads take more:
+// the kernel runs one wave per SIMD, which the latency-bound stream (its
+// longest particle's serial steps) does not miss.
 struct EvalMilkyWayTracked {
   static constexpr bool kLarge = false;
   static constexpr bool kFrames = true;
   static constexpr bool kInlineFirstStep = false;
-  static constexpr int kWavesPerSimd = 2;  // occupancy target (0: none)
-  __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
-                                             double z, double t, double& ax,
-                                             double& ay, double& az) {
+  static constexpr bool kTrackPrefetch = true;
+  static constexpr int kWavesPerSimd = 1;  // occupancy target (0: none)
+  template <int N>
+  __device__ __forceinline__ static void track_n(const DPot& P,
+                                                 const double (&t)[N],
+                                                 double (&d)[N][3]) {
+    track_pos_n<N>(P.tracks[P.comps[4].track], t, d);
+  }
+  __device__ __forceinline__ static void acc_at(const DPot& P, double x,
+                                                double y, double z, double t,
+                                                double dx, double dy, double dz,
+                                                double& ax, double& ay,
+                                                double& az) {
     EvalMilkyWay::acc(P, x, y, z, t, ax, ay, az);
     const DComp& c = P.comps[4];
-    double dx, dy, dz;
-    track_pos(P.tracks[c.track], t, dx, dy, dz);
     const double X = x - dx, Y = y - dy, Z = z - dz;
     const double id = rsqrt_fast(X * X + Y * Y + Z * Z + c.p[1]);
     const double g = c.p[0] * id * id * id;
     ax -= g * X;
     ay -= g * Y;
     az -= g * Z;
+  }
+  __device__ __forceinline__ static void acc(const DPot& P, double x, double y,
+                                             double z, double t, double& ax,
+                                             double& ay, double& az) {
+    double dx, dy, dz;
+    track_pos(P.tracks[P.comps[4].track], t, dx, dy, dz);
+    acc_at(P, x, y, z, t, dx, dy, dz, ax, ay, az);
   }
 };
 

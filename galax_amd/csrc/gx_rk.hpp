@@ -3,6 +3,8 @@
 // the shared-step integrators.
 #pragma once
 
+#include <type_traits>
+
 #include "gx_potential.hpp"
 #include "gx_tableau.hpp"
 
@@ -11,6 +13,36 @@ namespace gx {
 // Doubles per dense-output record of an accepted step: t, h, q, v and up to
 // 14 stage accelerations (the ABI's GX_DENSE_RECORD).
 constexpr int kDenseRec = 8 + 3 * 14;
+
+// Evaluators whose field rides a track and that look the track up at all of
+// a step's stage times when the step starts (E::kTrackPrefetch, with
+// E::track_n and E::acc_at).
+template <class E, class = void>
+struct track_prefetch : std::false_type {};
+template <class E>
+struct track_prefetch<E, std::void_t<decltype(E::kTrackPrefetch)>>
+    : std::bool_constant<E::kTrackPrefetch> {};
+
+// Track positions at stage times i0 + 1 .. i0 + N of a step (the end stage
+// at te), into d[i0 .. i0 + N - 1].
+template <class M, class E, int i0, int N>
+__device__ __forceinline__ void stage_tracks(const DPot& P, double dir,
+                                             double tb, double h, double te,
+                                             double (&d)[M::SF - 1][3]) {
+  double tt[N], dd[N][3];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const int i = i0 + 1 + j;
+    tt[j] = dir * ((i == M::SF - 1) ? te : fma(M::kf.c[i], h, tb));
+  }
+  E::template track_n<N>(P, tt, dd);
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    d[i0 + j][0] = dd[j][0];
+    d[i0 + j][1] = dd[j][1];
+    d[i0 + j][2] = dd[j][2];
+  }
+}
 
 // One step of method M from (tb, q, p) with step h to te = tb + h (solver
 // time tau; physical time dir * tau, velocities pre-multiplied by dir).
@@ -42,6 +74,16 @@ __device__ __forceinline__ void rk_step(const DPot& P, double dir, double tb,
   // Stage positions as q + rs (h p) + (h h) s: two FMAs per component
   const double hh = h * h;
   const double hp[3] = {h * p[0], h * p[1], h * p[2]};
+  // track positions at the stage times 1 .. SF-1 (slot i-1 for stage i),
+  // looked up in two batches of loads
+  constexpr bool kPre = track_prefetch<E>::value;
+  static_assert(!(kPre && E::kLarge), "track prefetch needs unrolled stages");
+  double trk[kPre ? SF - 1 : 1][3];
+  if constexpr (kPre) {
+    constexpr int NA = SF / 2;
+    stage_tracks<M, E, 0, NA>(P, dir, tb, h, te, trk);
+    stage_tracks<M, E, NA, SF - 1 - NA>(P, dir, tb, h, te, trk);
+  }
   if constexpr (!E::kLarge) {
 #pragma unroll
     for (int i = 1; i < S; ++i) {
@@ -66,7 +108,18 @@ __device__ __forceinline__ void rk_step(const DPot& P, double dir, double tb,
         Q2 = fma(hh, s2, Q2);
       }
       const double ts = (i == SF - 1) ? te : fma(M::kf.c[i], h, tb);
-      E::acc(P, Q0, Q1, Q2, dir * ts, g[i][0], g[i][1], g[i][2]);
+      if constexpr (kPre)
+        E::acc_at(P, Q0, Q1, Q2, dir * ts, trk[i - 1][0], trk[i - 1][1],
+                  trk[i - 1][2], g[i][0], g[i][1], g[i][2]);
+      else
+        E::acc(P, Q0, Q1, Q2, dir * ts, g[i][0], g[i][1], g[i][2]);
+    }
+    if constexpr (kPre && SF > S) {
+      // the end stage's track position waits in its acceleration slot
+      // until rk_end_stage evaluates it
+      g[SF - 1][0] = trk[SF - 2][0];
+      g[SF - 1][1] = trk[SF - 2][1];
+      g[SF - 1][2] = trk[SF - 2][2];
     }
   } else {
     double Qx = q[0], Qy = q[1], Qz = q[2];
@@ -160,14 +213,20 @@ __device__ __forceinline__ void rk_step(const DPot& P, double dir, double tb,
 }
 
 // The end stage a(q1, te) into slot SF-1 when it is not one of the
-// method's own stages (Dopri8).
+// method's own stages (Dopri8).  For a track-prefetching evaluator the slot
+// holds the track position at te that rk_step looked up.
 template <class M, class E>
 __device__ __forceinline__ void rk_end_stage(const DPot& P, double dir,
                                              double te, const double (&q1)[3],
                                              double (&g)[M::SF][3]) {
-  if constexpr (M::SF > M::S)
-    E::acc(P, q1[0], q1[1], q1[2], dir * te, g[M::SF - 1][0],
-           g[M::SF - 1][1], g[M::SF - 1][2]);
+  if constexpr (M::SF > M::S) {
+    constexpr int l = M::SF - 1;
+    if constexpr (track_prefetch<E>::value)
+      E::acc_at(P, q1[0], q1[1], q1[2], dir * te, g[l][0], g[l][1], g[l][2],
+                g[l][0], g[l][1], g[l][2]);
+    else
+      E::acc(P, q1[0], q1[1], q1[2], dir * te, g[l][0], g[l][1], g[l][2]);
+  }
 }
 
 
