@@ -170,11 +170,79 @@ class RotatingTranslation:
         return np.stack([x0 * c - y0 * s, x0 * s + y0 * c, np.broadcast_to(z0, np.shape(t))], axis=-1)
 
 
+def not_a_knot_slopes(t, y):
+    """First derivatives at the knots of the not-a-knot cubic spline through
+    (t, y) (y: (n, ...) values), the spline interpax's and SciPy's
+    CubicSpline build by default.  On each interval that spline is the cubic
+    Hermite interpolant of the end values and these slopes.
+
+    Slopes s_i satisfy, for the interior knots,
+      h_i s_(i-1) + 2 (h_(i-1) + h_i) s_i + h_(i-1) s_(i+1)
+          = 3 (h_i d_(i-1) + h_(i-1) d_i),
+    with h_i = t_(i+1) - t_i and d_i = (y_(i+1) - y_i) / h_i, and at each end
+    the not-a-knot condition (one cubic over the first two and over the last
+    two intervals).  Each end row is combined with its neighbour so that the
+    remaining system is strictly diagonally dominant and solved without
+    pivoting (Thomas algorithm); n = 2 gives the line and n = 3 the parabola
+    through the three points."""
+    t = np.asarray(t, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    n = t.size
+    if n < 2:
+        return np.zeros_like(y)
+    h = np.diff(t)
+    d = np.diff(y, axis=0) / h.reshape((-1,) + (1,) * (y.ndim - 1))
+    if n == 2:
+        return np.stack([d[0], d[0]])
+    if n == 3:
+        # the parabola through the three points
+        c2 = (d[1] - d[0]) / (t[2] - t[0])
+        s0 = d[0] - c2 * h[0]
+        return np.stack([s0, s0 + 2 * c2 * h[0], s0 + 2 * c2 * (t[2] - t[0])])
+    bshape = (-1,) + (1,) * (y.ndim - 1)
+    hb = lambda a: a.reshape(bshape)
+    # rows 1 .. n-2 of the tridiagonal system: lower a, diagonal b, upper c, rhs r
+    a = hb(h[1:].copy())                  # coefficient of s_(i-1), i = 1 .. n-2
+    b = hb(2.0 * (h[:-1] + h[1:]))
+    c = hb(h[:-1].copy())                 # coefficient of s_(i+1)
+    r = 3.0 * (hb(h[1:]) * d[:-1] + hb(h[:-1]) * d[1:])
+    a, b, c = (np.broadcast_to(x, r.shape).copy() for x in (a, b, c))
+    # left end: h1 s0 + (h0 + h1) s1 = r0; row 1 minus it eliminates s0
+    h0, h1 = h[0], h[1]
+    r0 = ((h0 + 2.0 * (h0 + h1)) * h1 * d[0] + h0 * h0 * d[1]) / (h0 + h1)
+    b[0] -= (h0 + h1)
+    r[0] -= r0
+    a[0] = 0.0
+    # right end: (hm + hn) s_(n-2) + hn s_(n-1) = rn with hn = h_(n-3),
+    # hm = h_(n-2); row n-2 minus (h_(n-3)/hn) times it eliminates s_(n-1)
+    hn, hm = h[-2], h[-1]
+    rn = ((hm + 2.0 * (hm + hn)) * hn * d[-1] + hm * hm * d[-2]) / (hm + hn)
+    b[-1] -= (hm + hn)
+    r[-1] -= rn
+    c[-1] = 0.0
+    # Thomas algorithm over rows 1 .. n-2
+    m = r.shape[0]
+    for i in range(1, m):
+        w = a[i] / b[i - 1]
+        b[i] = b[i] - w * c[i - 1]
+        r[i] = r[i] - w * r[i - 1]
+    s_in = np.empty_like(r)
+    s_in[-1] = r[-1] / b[-1]
+    for i in range(m - 2, -1, -1):
+        s_in[i] = (r[i] - c[i] * s_in[i + 1]) / b[i]
+    s0 = (r0 - (h0 + h1) * s_in[0]) / h1
+    sn = (rn - (hm + hn) * s_in[-1]) / hn
+    return np.concatenate([s0[None], s_in, sn[None]])
+
+
 class TimeDependentTranslationParameter:
-    """Tabulated centre track.  The reference fits an interpax cubic spline to
-    positions (xfm/translate.py:185-199); here the track is a cubic Hermite
-    interpolant of positions and velocities at the knots.  Without
-    velocities, they are estimated with second-order finite differences."""
+    """Tabulated centre track.  Given positions only, the track is the
+    not-a-knot cubic spline through them, as the reference's interpax
+    CubicSpline (xfm/translate.py:185-199); given velocities as well, it is
+    the cubic Hermite interpolant of positions and velocities at the knots
+    (what the mock-stream generator passes: the progenitor's own states).
+    Outside the knot range the track holds its end positions, where the
+    reference's spline (extrapolate=False) gives NaN."""
 
     def __init__(self, t, q, v=None):
         t = np.asarray(ustrip(t, "Myr") if isinstance(t, Q) else t, dtype=np.float64)
@@ -182,7 +250,7 @@ class TimeDependentTranslationParameter:
         order = np.argsort(t)
         t, q = t[order], q[order]
         if v is None:
-            v = np.gradient(q, t, axis=0, edge_order=2)
+            v = not_a_knot_slopes(t, q)
         else:
             v = np.asarray(ustrip(v, "kpc/Myr") if isinstance(v, Q) else v, dtype=np.float64)[order]
         self.track = Track(t=t, q=q, v=v)
