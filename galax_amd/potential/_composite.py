@@ -259,6 +259,19 @@ class TimeDependentTranslationParameter:
     def from_(cls, t, q, v=None):
         return cls(t, q, v)
 
+    @classmethod
+    def from_callable(cls, fn, t):
+        """A callable translation t -> (x, y, z) (the reference accepts any
+        callable, translate.py:165-176) tabulated at the times t and
+        interpolated by the not-a-knot cubic spline through those values:
+        the device evaluates tracks, not Python callables, so the caller
+        chooses the knots that resolve the motion."""
+        t = np.asarray(ustrip(t, "Myr") if isinstance(t, Q) else t, dtype=np.float64).reshape(-1)
+        q = np.asarray(fn(t), dtype=np.float64)
+        if q.shape != (t.size, 3):
+            raise ValueError(f"the translation callable must map times (n,) to positions (n, 3); got {q.shape}")
+        return cls(t, q)
+
 
 class TranslatedPotential(AbstractPotential):
     """Phi_base(x - Delta(t)).  `translation` is a constant offset (3,), a
@@ -281,6 +294,10 @@ class TranslatedPotential(AbstractPotential):
                 rows.append((row, None))
             elif isinstance(tr, TimeDependentTranslationParameter):
                 rows.append((row, tr.track))
+            elif callable(tr):
+                raise TypeError(
+                    "a callable translation runs in Python; tabulate it on the times that resolve "
+                    "it with TimeDependentTranslationParameter.from_callable(fn, t)")
             else:
                 off = np.asarray(ustrip(tr, "kpc") if isinstance(tr, Q) else tr, dtype=np.float64)
                 if off.shape != (3,):

@@ -79,3 +79,18 @@ def test_spline_track_on_gpu_matches_scipy_and_oracle():
     shift = scipy_interp.CubicSpline(t, y)(tq)
     want = oracle.evaluate(base, "acceleration", x - shift)
     np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-12 * np.abs(want).max())
+
+
+def test_callable_translation_is_tabulated_on_given_knots():
+    fn = lambda t: np.stack([np.cos(0.05 * t), np.sin(0.05 * t), 0.01 * t], -1) * 7.0
+    t = np.linspace(-100.0, 100.0, 201)
+    tr = gp.TimeDependentTranslationParameter.from_callable(fn, t)
+    tq = np.random.default_rng(1).uniform(-100.0, 100.0, 300)
+    np.testing.assert_allclose(_hermite(tr.track.t, tr.track.q, tr.track.v, tq),
+                               scipy_interp.CubicSpline(t, fn(t))(tq), rtol=0, atol=1e-11)
+    # a callable passed straight to TranslatedPotential is refused with the remedy
+    pot = gp.TranslatedPotential(gp.PlummerPotential(m_tot=1e9, r_s=0.1), fn)
+    with pytest.raises(TypeError, match="from_callable"):
+        pot._rows()
+    with pytest.raises(ValueError, match="positions"):
+        gp.TimeDependentTranslationParameter.from_callable(lambda t: t, t)
