@@ -316,6 +316,9 @@ def main() -> int:
     ap.add_argument("--single-launch", type=int, default=3,
                     help="after the timed region, passes timed as one launch on the whole device (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="surrogate workload: launch the lock-step kernels one by one instead of replaying a "
+                         "captured step (for profilers that do not follow graph replays)")
     ap.add_argument(
         "--workload",
         choices=("uneven", "uniform", "longest", "copy", "surrogate", "streamfit", "stream", "c1"),
@@ -374,7 +377,8 @@ def main() -> int:
     if args.workload == "surrogate":
         from galax_amd import surrogate
 
-        rep = surrogate.barred_halo_experiment(log=lambda m: print(m, file=sys.stderr, flush=True))
+        rep = surrogate.barred_halo_experiment(use_graph=not args.no_graph,
+                                               log=lambda m: print(m, file=sys.stderr, flush=True))
         print(json.dumps({"report": "learned-surrogate field vs exact FP64 (barred MW + 64 subhaloes)", **rep}),
               flush=True)
         return 0

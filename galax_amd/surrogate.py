@@ -419,7 +419,7 @@ def _position_error(w, ref):
 
 
 def barred_halo_experiment(n_train: int = 262_144, n_bench: int = 262_144, t1: float = 1000.0, dt: float = 1.0,
-                           epochs: int = 8, ref_rtol: float = 1e-8, log=None):
+                           epochs: int = 8, ref_rtol: float = 1e-8, use_graph: bool = True, log=None):
     """Trains the surrogate on exact orbits (seed 4) and integrates new
     orbits (seed 5) over t1 with (a) the surrogate (lock-step fixed step dt,
     gate on), (b) the exact FP64 field at the same fixed step, and (c) the
@@ -427,7 +427,9 @@ def barred_halo_experiment(n_train: int = 262_144, n_bench: int = 262_144, t1: f
     ref_rtol, whose Jacobi-energy drift sits at round-off.  Position errors of (a) and (b)
     are measured against (c); so is the Milky Way model alone (bar and
     subhaloes ignored), which is what the network is meant to improve on.
-    Times are HIP events around each whole integration."""
+    Times are HIP events around each whole integration.  use_graph=False
+    launches the lock-step kernels one by one instead of replaying a
+    captured step (for tools that do not follow graph replays)."""
     torch = _lib.require_gpu()
     from . import workloads
     from .dynamics import ConstantStepSize, Dopri8, OrbitSolver, PIDController, integrate_orbits
@@ -453,7 +455,7 @@ def barred_halo_experiment(n_train: int = 262_144, n_bench: int = 262_144, t1: f
     res_fixed, ms_exact = timed(lambda: integrate_orbits(full, w0, 0.0, t1, solver=fixed, dt0=dt))
     res_ref, ms_ref = timed(lambda: integrate_orbits(full, w0, 0.0, t1, solver=ref_solver))
     res_mw, _ = timed(lambda: integrate_orbits(base, w0, 0.0, t1, solver=ref_solver))
-    (w_sur, sinfo), _ = timed(lambda: integrate_lockstep(field, w0, 0.0, t1, dt))
+    (w_sur, sinfo), ms_sur_call = timed(lambda: integrate_lockstep(field, w0, 0.0, t1, dt, use_graph=use_graph))
     ms_sur = sinfo["ms_timed"] * sinfo["steps"] / max(sinfo["steps_timed"], 1)
     ref = res_ref.out[:, -1, :]
     w_exact = res_fixed.out[:, -1, :]
@@ -482,7 +484,9 @@ def barred_halo_experiment(n_train: int = 262_144, n_bench: int = 262_144, t1: f
         "exact_fixed_ms": ms_exact,
         "surrogate_ms": ms_sur,
         "surrogate_ms_basis": "lock-step graph replays timed with HIP events; the first step (graph capture) "
-                              "is scaled in at the mean replay time",
+                              "is scaled in at the mean replay time" if use_graph else
+                              "lock-step launches timed with HIP events (no graph)",
+        "surrogate_ms_whole_call": ms_sur_call,
         "exact_fixed_orbit_steps_per_s": steps / (ms_exact * 1e-3),
         "surrogate_orbit_steps_per_s": steps / (ms_sur * 1e-3),
         "surrogate_speedup_vs_exact_fixed_step": ms_exact / ms_sur,
