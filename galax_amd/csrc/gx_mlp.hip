@@ -302,7 +302,11 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs f) {
 }
 
 // db[n] += sum_m dZ[m, n] (bf16 in, fp32 out) -- partial column sums per
-// block combined with atomics.
+// block of kColsumRows rows, combined with atomics.  A lane keeps four
+// independent sums so that four rows' loads are in flight at once; the
+// grid has one block row per chunk of rows, enough to fill the device for
+// a training batch (65,536 rows: 512 chunks).
+constexpr int kColsumRows = 128;
 __global__ void k_colsum(const __bf16* __restrict__ dz, long long ld, long long M,
                          long long N, float* __restrict__ out) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -310,9 +314,16 @@ __global__ void k_colsum(const __bf16* __restrict__ dz, long long ld, long long 
   const long long rows = (M + gridDim.y - 1) / gridDim.y;
   const long long r0 = (long long)blockIdx.y * rows;
   const long long r1 = r0 + rows < M ? r0 + rows : M;
-  float s = 0.f;
-  for (long long m = r0; m < r1; ++m) s += (float)dz[m * ld + n];
-  atomicAdd(out + n, s);
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  long long m = r0;
+  for (; m + 3 < r1; m += 4) {
+    s0 += (float)dz[m * ld + n];
+    s1 += (float)dz[(m + 1) * ld + n];
+    s2 += (float)dz[(m + 2) * ld + n];
+    s3 += (float)dz[(m + 3) * ld + n];
+  }
+  for (; m < r1; ++m) s0 += (float)dz[m * ld + n];
+  atomicAdd(out + n, (s0 + s1) + (s2 + s3));
 }
 
 // Adam on fp32 master parameters (bias-corrected, as Kingma & Ba 2015);
@@ -594,8 +605,8 @@ int gx_mlp_colsum(const void* dz, int64_t ld, int64_t M, int64_t N, float* out,
                   void* stream) {
   if (!dz || !out || M < 0 || N < 0 || ld < N) return GX_EINVAL;
   if (M == 0 || N == 0) return GX_OK;
-  long long ychunks = M / 1024 + 1;
-  if (ychunks > 1024) ychunks = 1024;
+  long long ychunks = (M + kColsumRows - 1) / kColsumRows;
+  if (ychunks > 65535) ychunks = 65535;  // grid y limit; chunks then grow
   dim3 grid((unsigned)blocks_for(N, 256), (unsigned)ychunks);
   hipLaunchKernelGGL(k_colsum, grid, dim3(256), 0, (hipStream_t)stream,
                      static_cast<const __bf16*>(dz), (long long)ld, (long long)M,

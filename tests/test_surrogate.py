@@ -165,6 +165,26 @@ def test_adam_matches_formula():
     np.testing.assert_allclose(pb.double().cpu().numpy(), P, rtol=2.0 ** -8, atol=1e-6)
 
 
+@pytest.mark.parametrize("M", [1, 7, 1000, 65536 + 3])
+def test_column_sums_match_torch(M):
+    """gx_mlp_colsum (bias gradients): column sums of a BF16 matrix with a
+    row stride, accumulated onto the output, against float64 sums; rows
+    not a multiple of the kernel's chunk or of its four-row unroll."""
+    torch = _torch()
+    from galax_amd import _lib
+    import galax_amd.surrogate as S
+
+    g = torch.Generator().manual_seed(M)
+    N, ld = 67, 72
+    A = torch.randn(M, ld, generator=g).to(torch.bfloat16)
+    out = torch.full((N,), 0.25, dtype=torch.float32, device="cuda")
+    Ad = A.cuda()
+    _lib.check(_lib.lib().gx_mlp_colsum(S._p(Ad), ld, M, N, S._p(out), S._s()), "gx_mlp_colsum")
+    want = 0.25 + A[:, :N].double().sum(0)
+    got = out.cpu().double()
+    assert torch.allclose(got, want, rtol=1e-5, atol=1e-5 * (M ** 0.5)), (got - want).abs().max()
+
+
 def test_mlp_gradients_match_autograd():
     """Gradients of the pseudo-Huber loss from the library's backward pass
     against torch autograd in float32 on the same BF16 weights; activations
