@@ -75,6 +75,42 @@ def _generator(rng):
     return np.random.Generator(np.random.PCG64(int(rng)))
 
 
+def release_torch(x, v, r_t, k):
+    """Fardal (2015) leading/trailing release states, as
+    FardalStreamDF.release, in torch operations (differentiable, and on the
+    device when the inputs are): rows [lead; trail], each (x, v)."""
+    import torch
+
+    r = x.norm(dim=-1, keepdim=True)
+    L = torch.linalg.cross(x, v, dim=-1)
+    om = L.norm(dim=-1, keepdim=True) / r**2
+    r_hat = x / r
+    z_hat = L / L.norm(dim=-1, keepdim=True)
+    phi_vec = v - (v * r_hat).sum(-1, keepdim=True) * r_hat
+    phi_hat = phi_vec / phi_vec.norm(dim=-1, keepdim=True)
+    rt = r_t[:, None]
+    vc = om * rt
+    kr, kvphi, kz, kvz = (k[:, i : i + 1] for i in range(4))
+    x_trail = x + rt * (kr * r_hat + kz * z_hat)
+    v_trail = v + vc * (kvphi * phi_hat + kvz * z_hat)
+    x_lead = x - rt * (kr * r_hat - kz * z_hat)
+    v_lead = v - vc * (kvphi * phi_hat - kvz * z_hat)
+    return torch.cat([torch.cat([x_lead, v_lead], -1), torch.cat([x_trail, v_trail], -1)], 0)
+
+
+def tidal_radius_device(pot, x, v, mass, t):
+    """King (1962) tidal radius at device positions/velocities (torch, FP64),
+    the radial second derivative from the device Hessian (radius.py:200-215)."""
+    import torch
+
+    H = pot.hessian(x, t)
+    r_hat = x / x.norm(dim=-1, keepdim=True)
+    d2 = (r_hat[:, :, None] * H * r_hat[:, None, :]).sum(dim=(1, 2))
+    L = torch.linalg.cross(x, v, dim=-1)
+    om = L.norm(dim=-1) / (x * x).sum(-1)
+    return torch.pow(G * mass / (om * om - d2), 1.0 / 3.0)
+
+
 class FardalStreamDF:
     """Fardal+2015 stream distribution function."""
 
@@ -196,10 +232,25 @@ class MockStreamGenerator:
         prog_o = evaluate_orbit(self.potential, w0, ts, integrator=self.progenitor_integrator)
         prog_o.q = np.asarray(prog_o.q).reshape(-1, 3)
         prog_o.p = np.asarray(prog_o.p).reshape(-1, 3)
-        mock0 = self.df.sample(rng, self.potential, prog_o, prog_mass)
         t_f = ts[-1] + 1e-3
         n = ts.size
-        w_all = np.concatenate([mock0["lead"].w(), mock0["trail"].w()])
+        if type(self.df) is FardalStreamDF:
+            # the release on the device: King radius from the device Hessian,
+            # Fardal's formula in torch; only the k draws come from the host
+            # generator (the same draws FardalStreamDF.sample makes)
+            import torch
+
+            dev = dict(device="cuda", dtype=torch.float64)
+            x = torch.as_tensor(prog_o.q, **dev)
+            v = torch.as_tensor(prog_o.p, **dev)
+            tt = torch.as_tensor(ts, **dev)
+            mass = prog_mass(ts) if callable(prog_mass) else np.full(ts.shape, float(ustrip(prog_mass, "Msun")))
+            r_t = tidal_radius_device(self.potential, x, v, torch.as_tensor(np.asarray(mass, np.float64), **dev), tt)
+            k = torch.as_tensor(self.df.draw_k(rng, n), **dev)
+            w_all = release_torch(x, v, r_t, k)
+        else:
+            mock0 = self.df.sample(rng, self.potential, prog_o, prog_mass)
+            w_all = np.concatenate([mock0["lead"].w(), mock0["trail"].w()])
         t0_all = np.concatenate([ts, ts])
         pot = self.stream_potential(prog_o)
         integ = self.stream_integrator

@@ -32,38 +32,13 @@ import numpy as np
 
 from . import _lib
 from .adjoint import FlowSpec, MilkyWayParams, OrbitFlow
+from .dynamics._mockstream import release_torch
 
 __all__ = ["StreamFitProblem", "make_problem", "stream_loss", "release_torch", "mw_d2phi_dr2", "tidal_radius_torch"]
 
 # galax's MilkyWayPotential nucleus (Hernquist), not among the fitted
 # parameters
 NUCLEUS_M, NUCLEUS_C = 1.71e9, 0.07
-
-
-def _cross(a, b):
-    torch = _lib.torch()
-    return torch.linalg.cross(a, b, dim=-1)
-
-
-def release_torch(x, v, r_t, k):
-    """Fardal (2015) leading/trailing release states, as
-    FardalStreamDF.release, in differentiable torch operations."""
-    torch = _lib.torch()
-    r = x.norm(dim=-1, keepdim=True)
-    L = _cross(x, v)
-    om = L.norm(dim=-1, keepdim=True) / r**2
-    r_hat = x / r
-    z_hat = L / L.norm(dim=-1, keepdim=True)
-    phi_vec = v - (v * r_hat).sum(-1, keepdim=True) * r_hat
-    phi_hat = phi_vec / phi_vec.norm(dim=-1, keepdim=True)
-    rt = r_t[:, None]
-    vc = om * rt
-    kr, kvphi, kz, kvz = (k[:, i : i + 1] for i in range(4))
-    x_trail = x + rt * (kr * r_hat + kz * z_hat)
-    v_trail = v + vc * (kvphi * phi_hat + kvz * z_hat)
-    x_lead = x - rt * (kr * r_hat - kz * z_hat)
-    v_lead = v - vc * (kvphi * phi_hat - kvz * z_hat)
-    return torch.cat([torch.cat([x_lead, v_lead], -1), torch.cat([x_trail, v_trail], -1)], 0)
 
 
 def mw_d2phi_dr2(x, theta):
