@@ -81,3 +81,10 @@ def test_noalloc_refuses_batches_beyond_its_capacity():
                                   None, 1, _abi.GX_TS_FINAL, cap + 1, None, _lib.ptr(o), None, None, None,
                                   _lib.stream_ptr())
     assert rc == _abi.GX_ETOOBIG
+    # an empty batch is a no-op and a negative size an argument error, as
+    # for gx_integrate
+    args = (pot._handle(), ctypes.addressof(cfg), _lib.ptr(w), _lib.ptr(t), _lib.ptr(t), 0, None, 1,
+            _abi.GX_TS_FINAL)
+    tail = (None, _lib.ptr(o), None, None, None, _lib.stream_ptr())
+    assert lib.gx_integrate_noalloc(*args, 0, *tail) == _abi.GX_OK
+    assert lib.gx_integrate_noalloc(*args, -1, *tail) == _abi.GX_EINVAL
