@@ -23,9 +23,12 @@ from pathlib import Path
 
 # Hardware queues of this process (HIP reads the variable when it starts):
 # the batches in flight each need their own queue to run side by side, so
-# the process asks for eight when its environment offers fewer.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or "4") < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+# the process asks for sixteen when its environment offers fewer.  With
+# eight, eight batches in flight shared queues and a pass took 30 ms; with
+# sixteen, 8 in flight ran at 18.7-18.9 ms against 19.2-19.4 for 6 in
+# flight on eight queues.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or "4") < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -295,10 +298,10 @@ def streamfit_report(steps: int) -> dict:
     }
 
 
-# Independent batches kept in flight by the default throughput run (each on
-# its own stream, each launch taking a quarter of the device; measured best
-# with eight hardware queues: 4 and 12 in flight within 5 %).
-IN_FLIGHT = 6
+# Independent batches kept in flight when the launch policy is not tuned
+# (each on its own stream, each launch taking a quarter of the device):
+# the tuner's starting point, measured best with sixteen hardware queues.
+IN_FLIGHT = 8
 
 
 def main() -> int:

@@ -10,7 +10,9 @@ queue at a time.  `tune_batches` searches these by coordinate descent
 (vary one setting at a time from the current best, keep any change that is
 faster by more than the noise margin, repeat until a sweep changes nothing),
 timing short pipelined runs of the caller's own batch.  The result is cached
-per device, library build and workload, so a process tunes once.
+per device, library build and workload, so a process tunes once.  Batches in
+flight beyond the process's hardware queues (GPU_MAX_HW_QUEUES, read when
+HIP starts) share queues and run in turn, which the measurements show.
 
 Orbits per lane is one by design (the state, stages and controller of an
 orbit fill a lane's registers), and waves per SIMD is fixed at compile time
@@ -32,13 +34,13 @@ from . import _lib
 # the library, so 1 and 2 (2 = the Milky Way kernel's occupancy at 256 lanes
 # per workgroup) cover it.
 SPACE = {
-    "in_flight": (1, 2, 3, 4, 6, 8),
+    "in_flight": (1, 2, 3, 4, 6, 8, 10, 12),
     "parts": (1, 2, 3, 4, 6),
     "blocks_per_cu": (1, 2),
     "head_lanes": (0, 1024, 4096, 8192),
     "chunk": (1, 2, 4),
 }
-START = {"in_flight": 6, "parts": 4, "blocks_per_cu": 2, "head_lanes": 4096, "chunk": 1}
+START = {"in_flight": 8, "parts": 4, "blocks_per_cu": 2, "head_lanes": 4096, "chunk": 1}
 MAX_IN_FLIGHT = max(SPACE["in_flight"])
 
 
