@@ -32,9 +32,11 @@ from . import _lib
 
 # The search space.  blocks_per_cu above the kernel's occupancy is clipped by
 # the library, so 1 and 2 (2 = the Milky Way kernel's occupancy at 256 lanes
-# per workgroup) cover it.
+# per workgroup) cover it.  Twelve batches in flight (three devices' worth of
+# quarter-device launches) measured 19.0 ms per pass in one run and 23.7 in
+# another on sixteen hardware queues, so the search stops at ten.
 SPACE = {
-    "in_flight": (1, 2, 3, 4, 6, 8, 10, 12),
+    "in_flight": (1, 2, 3, 4, 6, 8, 10),
     "parts": (1, 2, 3, 4, 6),
     "blocks_per_cu": (1, 2),
     "head_lanes": (0, 1024, 4096, 8192),
@@ -100,11 +102,15 @@ def _store_cache(c: dict) -> None:
 
 
 def time_batches(torch, launch, settings: dict, passes: int, warmup: int) -> float:
-    """Wall milliseconds per pass of `passes` batches kept `in_flight` deep
-    on as many streams, after `warmup` untimed passes; launch(j) issues one
-    pass on the current stream using buffer set j."""
+    """Wall milliseconds per pass of at least `passes` batches (and three per
+    stream, so that the pipeline reaches its steady state rather than
+    timing one round of launches) kept `in_flight` deep on as many streams,
+    after at least `warmup` (and one per stream) untimed passes; launch(j)
+    issues one pass on the current stream using buffer set j."""
     apply(settings)
     nfl = int(settings["in_flight"])
+    passes = max(passes, 3 * nfl)
+    warmup = max(warmup, nfl)
     streams = [torch.cuda.Stream() for _ in range(nfl)]
     main = torch.cuda.current_stream()
     for s in streams:
@@ -195,7 +201,8 @@ def tune_batches(torch, launch, workload_key: str, *, max_in_flight: int, passes
         "seconds": time.perf_counter() - t_start,
         "cached": False,
         "search": "coordinate descent over in_flight x parts x blocks_per_cu x head_lanes x chunk from "
-                  f"{start}, {passes} timed passes per trial, keep a change only if > {margin:.0%} faster",
+                  f"{start}, max({passes}, 3 x in_flight) timed passes per trial, keep a change only if > "
+                  f"{margin:.0%} faster",
         "fixed": {"orbits_per_lane": 1, "waves_per_simd": "compile-time (register budget)"},
     }
     if use_cache:
