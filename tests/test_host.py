@@ -237,7 +237,7 @@ def _kernel_resources(tmp_path):
     return out
 
 
-def test_integrator_kernels_keep_two_waves_per_simd(tmp_path):
+def test_integrator_kernels_keep_their_occupancy(tmp_path):
     """The Milky Way and frame-less table integrators stay within 256 VGPRs
     (two waves per SIMD; above it a SIMD holds one wave of them and the
     throughput halves): a guard against additions to the field code that
