@@ -434,11 +434,16 @@ def main() -> int:
 
     tuning = None
     explicit = args.in_flight is not None or args.launch_parts is not None or args.blocks_per_cu is not None
+    # The streams are created once, before any timing: HIP maps each new
+    # stream onto one of the process's hardware queues in turn, and streams
+    # created per tuning trial left the timed run's streams sharing queues
+    # (28 ms per pass instead of 19).
+    from galax_amd import tune
+
+    all_streams = [torch.cuda.Stream() for _ in range(max(tune.MAX_IN_FLIGHT, args.in_flight or 1))]
     if args.workload == "uneven" and not explicit and not args.no_tune:
         # the library's launch policy for this batch, searched on this device
         # (cached per device, build and workload)
-        from galax_amd import tune
-
         outs, wss = buffers(tune.MAX_IN_FLIGHT)
 
         def launch(j):
@@ -446,7 +451,7 @@ def main() -> int:
                                 out=outs[j], workspace=wss[j])
 
         tuning = tune.tune_batches(torch, launch, f"uneven:n={n}:saves={args.saves}:rtol=1e-8",
-                                   max_in_flight=tune.MAX_IN_FLIGHT,
+                                   max_in_flight=tune.MAX_IN_FLIGHT, streams=all_streams,
                                    log=lambda m: print(m, file=sys.stderr, flush=True))
         nfl = int(tuning["settings"]["in_flight"])
         parts = int(tuning["settings"]["parts"])
@@ -457,7 +462,7 @@ def main() -> int:
         parts = args.launch_parts if args.launch_parts is not None else min(nfl, 4)
         _lib.check(_lib.lib().gx_set_launch_fraction(parts, None), "gx_set_launch_fraction")
         outs, wss = buffers(nfl)
-    streams = [torch.cuda.Stream() for _ in range(nfl)]
+    streams = all_streams[:nfl]
 
     def run(k, launch_events=None):
         # one pass on stream k % nfl: gx_integrate's queue reset, first-step

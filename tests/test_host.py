@@ -281,6 +281,27 @@ def test_caller_buffers_are_checked():
         _check_out(torch, np.zeros((5, 3, 6)), 5, 3)
 
 
+def test_autotuner_keeps_the_start_unless_the_winner_wins_again():
+    """A setting that won the search on one noisy trial replaces the start
+    only if it is faster again, by the margin, in alternating re-measurements."""
+    from galax_amd.tune import START, confirm_winner
+
+    lucky = dict(START, in_flight=10)
+    calls = []
+
+    def flat(c):  # the winner's first trial was luck: both settings equal
+        calls.append(dict(c))
+        return 20.0
+
+    chosen, v, rec = confirm_winner(flat, START, lucky, 15.0, 0.02)
+    assert chosen == START and v == 20.0 and rec["kept"] == START
+    assert calls == [START, lucky, START, lucky]  # alternating
+    faster = lambda c: 18.0 if c["in_flight"] == 10 else 20.0
+    chosen, v, rec = confirm_winner(faster, START, lucky, 18.0, 0.02)
+    assert chosen == lucky and v == 18.0
+    assert confirm_winner(flat, START, dict(START), 19.0, 0.02) == (START, 19.0, None)
+
+
 def test_autotuner_search_finds_the_minimum_and_measures_each_setting_once():
     from galax_amd.tune import SPACE, START, coordinate_descent
 
