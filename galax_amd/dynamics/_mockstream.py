@@ -21,6 +21,10 @@ following the same distributions.
 With `progenitor_potential`, the particles also feel the progenitor: that
 potential is translated along the progenitor's own integrated track (cubic
 Hermite interpolation of its saved positions and velocities).
+
+`MockStreamGenerator.run` samples the release on the device (the King radius
+from the device Hessian, Fardal's formula in torch); `FardalStreamDF.sample`
+is the same computation in NumPy, for callers that want host arrays.
 """
 
 from __future__ import annotations
@@ -108,7 +112,8 @@ def tidal_radius_device(pot, x, v, mass, t):
     d2 = (r_hat[:, :, None] * H * r_hat[:, None, :]).sum(dim=(1, 2))
     L = torch.linalg.cross(x, v, dim=-1)
     om = L.norm(dim=-1) / (x * x).sum(-1)
-    return torch.pow(G * mass / (om * om - d2), 1.0 / 3.0)
+    a = G * mass / (om * om - d2)
+    return torch.sign(a) * torch.abs(a) ** (1.0 / 3.0)  # a real cube root, as np.cbrt
 
 
 class FardalStreamDF:

@@ -86,7 +86,8 @@ def tidal_radius_torch(x, v, theta, mass):
     torch = _lib.torch()
     L = torch.linalg.cross(x, v, dim=-1)
     om = L.norm(dim=-1) / (x * x).sum(-1)
-    return torch.pow(G * mass / (om * om - mw_d2phi_dr2(x, theta)), 1.0 / 3.0)
+    a = G * mass / (om * om - mw_d2phi_dr2(x, theta))
+    return torch.sign(a) * torch.abs(a) ** (1.0 / 3.0)  # a real cube root, as np.cbrt
 
 
 @dataclass
