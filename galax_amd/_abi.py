@@ -88,6 +88,9 @@ class GxTrack(ctypes.Structure):
         ("t", ctypes.c_void_p),
         ("q", ctypes.c_void_p),
         ("v", ctypes.c_void_p),
+        ("rec", ctypes.c_void_p),
+        ("method", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

@@ -260,11 +260,23 @@ __device__ __forceinline__ long long next_orbit(const IArgs& A, Cold& C,
   return ticket_to_orbit(A, idx);
 }
 
+// Evaluators that stage their track in the launch's dynamic LDS
+// (E::kLdsTrack, with E::stage).
+template <class E, class = void>
+struct lds_track : std::false_type {};
+template <class E>
+struct lds_track<E, std::void_t<decltype(E::kLdsTrack)>>
+    : std::bool_constant<E::kLdsTrack> {};
+
 template <class M, bool PID, class E, bool DENSE>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(E::kWavesPerSimd > 0 ? E::kWavesPerSimd : 1)))
     k_integrate(DPot P, IArgs A) {
   __shared__ Cold C;
+  if constexpr (lds_track<E>::value) {
+    E::stage(P);
+    __syncthreads();
+  }
   const int L = threadIdx.x;
   const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   // The first tickets of a cost-ordered queue hold the longest orbits, whose
@@ -485,8 +497,17 @@ int launch(const gx_pot* pot, IArgs& A, hipStream_t stream) {
   const auto kern = k_integrate<M, PID, E, DENSE>;
   const LaunchShape& ls = launch_shape();
   const int block = kBlock;
+  // dynamic LDS: the staged track of an LDS-track evaluator
+  size_t dyn = 0;
+  if constexpr (lds_track<E>::value) {
+    dyn = sizeof(double) * (size_t)pot->dev.trk.lds_doubles;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)dyn) != hipSuccess)
+      return GX_ELAUNCH;
+  }
   int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, dyn) !=
           hipSuccess ||
       occ < 1)
     occ = 1;
@@ -513,7 +534,7 @@ int launch(const gx_pot* pot, IArgs& A, hipStream_t stream) {
                        dim3((unsigned)fblocks), dim3(256), 0, stream, pot->dev,
                        A);
   }
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), 0, stream,
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(block), dyn, stream,
                      pot->dev, A);
   return check(hipGetLastError());
 }
@@ -705,6 +726,9 @@ int gx_integrate(const gx_pot* pot, const gx_solver_cfg* cfg,
     if (pot->sig == kSigMilkyWayTracked)
       return pid ? launch<Dopri8, true, EvalMilkyWayTracked>(pot, A, s)
                  : launch<Dopri8, false, EvalMilkyWayTracked>(pot, A, s);
+    if (pot->sig == kSigMilkyWayPieces)
+      return pid ? launch<Dopri8, true, EvalMilkyWayPieces>(pot, A, s)
+                 : launch<Dopri8, false, EvalMilkyWayPieces>(pot, A, s);
     if (pid)
       return pot->basic ? launch<Dopri8, true, Basic>(pot, A, s)
                         : launch<Dopri8, true, Full>(pot, A, s);

@@ -27,7 +27,8 @@ enum : int {
   kSigNone = 0,
   kSigMilkyWay = 1,
   kSigMilkyWayTracked = 2,
-  kSigMilkyWayRotating = 3
+  kSigMilkyWayRotating = 3,
+  kSigMilkyWayPieces = 4  // the mock-stream field on a dense-solution track
 };
 
 namespace gx {

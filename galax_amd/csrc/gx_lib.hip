@@ -2,9 +2,11 @@
 // libgalaxhip.so (declared in include/galax_amd.h).
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #include "gx_internal.hpp"
+#include "gx_tableau.hpp"
 
 namespace gx {
 
@@ -423,6 +425,112 @@ int field_args_ok(const gx_pot* pot, const void* xyz, const void* t,
   return 1;
 }
 
+// LDS that a launch may give a staged dense-solution track (pieces and
+// cells); the integrator's own shared state takes about 22 KB of the
+// workgroup's 160 KB.
+constexpr long long kLdsTrackBytes = 128 * 1024;
+
+// Pieces, cells and boundaries of a dense-solution track (the layout that
+// DTrack documents) from n accepted-step records of method M, in blob;
+// fills d's interval, end positions and cell geometry (not its pointers).
+// The records may run forward or backward in time; every step must have
+// the sign of the first, and each must start where the previous one ended
+// (to rounding).
+template <class M>
+int build_pieces(const gx_track& tr, std::vector<double>& blob, DTrack& d) {
+  constexpr int SF = M::SF;
+  constexpr int D = sizeof(M::dn.wq[0]) / sizeof(double);
+  static_assert(D + 1 <= 7, "pieces hold degree-6 polynomials");
+  const long long n = tr.n;
+  const double* R = tr.rec;
+  for (long long k = 0; k < n * GX_DENSE_RECORD; ++k)
+    if (!std::isfinite(R[k])) return GX_EINVAL;
+  const bool fwd = R[1] > 0.0;
+  std::vector<double> brk(n + 1), right(n);
+  std::vector<double> pieces(kPieceDoubles * n, 0.0);
+  for (long long j = 0; j < n; ++j) {
+    const double* r = R + GX_DENSE_RECORD * (fwd ? j : n - 1 - j);
+    const double ts = r[0], h = r[1];
+    if (!(fwd ? h > 0.0 : h < 0.0)) return GX_EINVAL;
+    double* pc = pieces.data() + kPieceDoubles * j;
+    pc[0] = ts;
+    pc[1] = 1.0 / h;
+    // q(th) = q + th h v + h^2 th sum_l Wq_l(th) g_l, Wq in Horner order
+    // (th^(D-1) first): the power of th of wq[l][e] is D - e
+    for (int m = 0; m < 3; ++m) {
+      double C[7] = {r[2 + m], h * r[5 + m], 0, 0, 0, 0, 0};
+      for (int e = 0; e < D; ++e) {
+        double acc = 0.0;
+        for (int l = 0; l < SF; ++l) acc += M::dn.wq[l][e] * r[8 + 3 * l + m];
+        C[D - e] += h * h * acc;
+      }
+      for (int e = 0; e < 7; ++e) pc[2 + 7 * m + e] = C[e];
+    }
+    brk[j] = fwd ? ts : ts + h;
+    right[j] = fwd ? ts + h : ts;
+  }
+  brk[n] = right[n - 1];
+  for (long long j = 0; j + 1 < n; ++j) {
+    // the end of piece j against the start of piece j + 1
+    const double tol = 64.0 * std::numeric_limits<double>::epsilon() *
+                       std::fmax(1.0, std::fabs(right[j]));
+    if (!(brk[j + 1] > brk[j]) || std::fabs(right[j] - brk[j + 1]) > tol)
+      return GX_EINVAL;
+  }
+  if (!(brk[n] > brk[n - 1])) return GX_EINVAL;
+  auto eval = [&](long long j, double t, double* q) {
+    const double* pc = pieces.data() + kPieceDoubles * j;
+    const double th = (t - pc[0]) * pc[1];
+    for (int m = 0; m < 3; ++m) {
+      const double* a = pc + 2 + 7 * m;
+      double v = a[6];
+      for (int e = 5; e >= 0; --e) v = std::fma(v, th, a[e]);
+      q[m] = v;
+    }
+  };
+  d.kind = kTrackPieces;
+  d.n = n;
+  d.t0 = brk[0];
+  d.t1 = brk[n];
+  eval(0, d.t0, d.q0);
+  eval(n - 1, d.t1, d.q1);
+  // cells: the coarsest table (2n, 4n, ... cells) in which every cell's
+  // times, widened by a millionth of a cell either side for rounding,
+  // cross at most two piece starts; none when LDS cannot hold one
+  const double inf = std::numeric_limits<double>::infinity();
+  const double span = d.t1 - d.t0;
+  std::vector<double> cells;
+  int ncell = 0;
+  for (long long nc = 2 * n; nc <= 64 * n; nc *= 2) {
+    if (8 * (kPieceDoubles * n + kCellDoubles * nc) > kLdsTrackBytes || nc > INT32_MAX) break;
+    cells.assign(kCellDoubles * nc, 0.0);
+    const double w = span / (double)nc, slack = 1e-6 * w;
+    bool ok = true;
+    long long k = 0;
+    for (long long c = 0; c < nc && ok; ++c) {
+      const double a = d.t0 + (double)c * w - slack, b = d.t0 + (double)(c + 1) * w + slack;
+      while (k + 1 < n && brk[k + 1] <= a) ++k;
+      ok = k + 3 >= n || brk[k + 3] > b;
+      double* e = cells.data() + kCellDoubles * c;
+      e[0] = (double)k;
+      e[1] = k + 1 < n ? brk[k + 1] : inf;
+      e[2] = k + 2 < n ? brk[k + 2] : inf;
+    }
+    if (ok) {
+      ncell = (int)nc;
+      break;
+    }
+  }
+  if (ncell == 0) cells.clear();
+  d.ncell = ncell;
+  d.inv_h = ncell > 0 ? (double)ncell / span : 0.0;
+  d.lds_doubles = ncell > 0 ? kPieceDoubles * n + kCellDoubles * (long long)ncell : 0;
+  blob = std::move(pieces);
+  blob.insert(blob.end(), cells.begin(), cells.end());
+  blob.insert(blob.end(), brk.begin(), brk.end());
+  return GX_OK;
+}
+
 }  // namespace
 }  // namespace gx
 
@@ -467,11 +575,24 @@ int gx_potential_create(const gx_component* comps, int32_t ncomp,
     dc[j - 1].flags |= kGroupEnd;
     i = j;
   }
-  // tracks: validate knots
+  // tracks: validate knots; build dense-solution tracks
   size_t knot_bytes = 0;
+  std::vector<std::vector<double>> blobs(ntracks > 0 ? ntracks : 1);
+  std::vector<DTrack> built(ntracks > 0 ? ntracks : 1);
   for (int k = 0; k < ntracks; ++k) {
     const gx_track& tr = tracks[k];
-    if (tr.n < 1 || !tr.t || !tr.q || !tr.v) return GX_EINVAL;
+    if (tr.reserved != 0) return GX_EINVAL;
+    if (tr.rec) {
+      if (tr.n < 1 || tr.n > INT32_MAX) return GX_EINVAL;
+      built[k] = DTrack{};
+      const int rc = tr.method == GX_DOPRI8   ? build_pieces<Dopri8>(tr, blobs[k], built[k])
+                     : tr.method == GX_DOPRI5 ? build_pieces<Dopri5>(tr, blobs[k], built[k])
+                                              : GX_EINVAL;
+      if (rc != GX_OK) return rc;
+      knot_bytes += align256(sizeof(double) * blobs[k].size());
+      continue;
+    }
+    if (tr.n < 1 || tr.n > INT32_MAX || !tr.t || !tr.q || !tr.v) return GX_EINVAL;
     for (int64_t i = 1; i < tr.n; ++i)
       if (!(tr.t[i] > tr.t[i - 1])) return GX_EINVAL;
     for (int64_t i = 0; i < tr.n; ++i)
@@ -503,7 +624,18 @@ int gx_potential_create(const gx_component* comps, int32_t ncomp,
   size_t off = comp_bytes + track_bytes;
   for (int k = 0; k < ntracks; ++k) {
     const gx_track& tr = tracks[k];
+    if (tr.rec) {
+      DTrack d = built[k];
+      std::memcpy(host.data() + off, blobs[k].data(), sizeof(double) * blobs[k].size());
+      d.rec = reinterpret_cast<const double*>(base + off);
+      d.cells = d.rec + kPieceDoubles * d.n;
+      d.brk = d.cells + kCellDoubles * (long long)d.ncell;
+      off += align256(sizeof(double) * blobs[k].size());
+      dt[k] = d;
+      continue;
+    }
     DTrack d{};
+    d.kind = kTrackKnots;
     d.n = tr.n;
     d.t0 = tr.t[0];
     d.t1 = tr.t[tr.n - 1];
@@ -567,8 +699,18 @@ int gx_potential_create(const gx_component* comps, int32_t ncomp,
   for (int i = 0; mw_origin && i < 4; ++i) mw_origin = comps[i].frame == GX_FRAME_NONE;
   if (mw_origin && comps[0].kind == GX_MIYAMOTO_NAGAI && comps[1].kind == GX_NFW &&
       comps[2].kind == GX_HERNQUIST && comps[3].kind == GX_HERNQUIST &&
-      comps[4].kind == GX_PLUMMER && comps[4].frame == GX_FRAME_TRACK)
-    h->sig = kSigMilkyWayTracked;
+      comps[4].kind == GX_PLUMMER && comps[4].frame == GX_FRAME_TRACK) {
+    // a knot track, or a dense-solution track that LDS holds with its
+    // cells (otherwise the table evaluator, which looks pieces up in
+    // device memory)
+    const DTrack& tk = dt[comps[4].track];
+    h->sig = tk.kind == kTrackKnots ? kSigMilkyWayTracked
+             : tk.ncell > 0        ? kSigMilkyWayPieces
+                                   : kSigNone;
+    h->dev.trk = tk;
+    h->dev.kp[0] = dc[4].p[0];
+    h->dev.kp[1] = dc[4].p[1];
+  }
   bool rot = mw_origin || ncomp > 5;
   for (int i = 0; rot && i < 4; ++i) rot = comps[i].frame == GX_FRAME_NONE;
   rot = rot && ncomp >= 5 && comps[0].kind == GX_MIYAMOTO_NAGAI && comps[1].kind == GX_NFW &&

@@ -52,18 +52,42 @@ struct DComp {
   double f[6];
 };
 
-// A track as one 128-byte record per knot interval k (t_k <= t < t_{k+1}):
+// A track of knots (kTrackKnots) as one 128-byte record per knot interval
+// k (t_k <= t < t_{k+1}):
 //   rec[16k + 0, 1, 2] = t_k, t_{k+1}, 1 / (t_{k+1} - t_k)
 //   rec[16k + 4 + 4m .. 7 + 4m] = power-basis coefficients c0..c3 of
 //   coordinate m of the cubic Hermite interpolant in s = (t - t_k)/(t_{k+1} - t_k)
 // so that a lookup touches one cache line and its loads are independent.
+//
+// A dense-solution track (kTrackPieces) as one 192-byte piece per accepted
+// step of the integrated orbit, in increasing time:
+//   rec[24k + 0, 1] = the step's start time t_s and 1/h (h signed)
+//   rec[24k + 2 + 7m .. 8 + 7m] = power-basis coefficients C0..C6 of
+//   coordinate m of the step's dense output in th = (t - t_s)/h
+// then `ncell` lookup cells of 4 doubles over [t0, t1] (cells = rec + 24n):
+//   (k, b1, b2, 0): every time that maps to the cell lies in piece
+//   k + (t >= b1) + (t >= b2), b1 and b2 the next two piece starts (+inf
+//   past the last piece)
+// then the n + 1 piece boundaries, increasing (brk).  The pieces and cells
+// are contiguous (lds_doubles of them), so that a kernel can stage them in
+// LDS with one copy; ncell = 0 when no cell table resolves every time.
+enum : int { kTrackKnots = 0, kTrackPieces = 1 };
+constexpr int kPieceDoubles = 24;
+constexpr int kCellDoubles = 4;
+
 struct DTrack {
   const double* rec;
-  long long n;      // knots (records: n - 1)
-  double t0, t1;    // first and last knot time
-  double inv_h;     // (n - 1) / (t1 - t0): guesses the interval
-  double q0[3];     // position held before the first knot
-  double q1[3];     // and after the last
+  long long n;      // knots (records: n - 1), or pieces
+  double t0, t1;    // covered interval
+  double inv_h;     // knots: (n - 1) / (t1 - t0), guesses the interval;
+                    // pieces: ncell / (t1 - t0), the cell of t
+  double q0[3];     // position held before t0
+  double q1[3];     // and after t1
+  int kind;
+  int ncell;
+  const double* cells;
+  const double* brk;
+  long long lds_doubles;  // pieces + cells, when ncell > 0
 };
 
 struct DPot {
@@ -72,6 +96,16 @@ struct DPot {
   const double* aux;  // per-component tables (quadrature nodes), by offset
   int n;
   int ntracks;
+  // By-value copies of what the mock-stream evaluator reads on every stage
+  // (its track and its Plummer row's p0, p1), filled for that composition
+  // only.  Kernel arguments are read-only for the launch, so a step loop
+  // keeps them in scalar registers; reads through `comps` and `tracks`
+  // (device memory that the loop's own stores could alias) are repeated on
+  // every trip, and a select between a track end and a looked-up position
+  // whose end value came from such a read compiled to a branch around a
+  // load.
+  DTrack trk;
+  double kp[2];
 };
 
 // ------------------------------------------------------------- tracks
@@ -93,8 +127,46 @@ __device__ __forceinline__ void load_track_record(const double* rec,
   }
 }
 
+// Position of piece k (a dense-solution track) at t: Horner in th.
+__device__ __forceinline__ void piece_pos(const double* r, double t,
+                                          double& x, double& y, double& z) {
+  const double th = (t - r[0]) * r[1];
+  double c[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const double* a = r + 2 + 7 * m;
+    double v = a[6];
+#pragma unroll
+    for (int d = 5; d >= 0; --d) v = fma(v, th, a[d]);
+    c[m] = v;
+  }
+  x = c[0];
+  y = c[1];
+  z = c[2];
+}
+
 __device__ inline void track_pos(const DTrack& T, double t, double& x,
                                  double& y, double& z) {
+  if (T.kind == kTrackPieces) {
+    if (t <= T.t0 || t >= T.t1) {
+      const double* q = t <= T.t0 ? T.q0 : T.q1;
+      x = q[0];
+      y = q[1];
+      z = q[2];
+      return;
+    }
+    // the last piece k with brk[k] <= t
+    long long lo = 0, hi = T.n - 1;
+    while (lo < hi) {
+      const long long mid = (lo + hi + 1) >> 1;
+      if (T.brk[mid] <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    piece_pos(T.rec + kPieceDoubles * lo, t, x, y, z);
+    return;
+  }
   if (T.n <= 1 || t <= T.t0) {
     x = T.q0[0];
     y = T.q0[1];
@@ -148,15 +220,28 @@ template <int N>
 __device__ __forceinline__ void track_pos_n(const DTrack& T,
                                             const double (&t)[N],
                                             double (&d)[N][3]) {
-  const long long last = T.n - 2;
+  // knots fit in 32 bits (the host refuses larger tracks)
+  const int last = (int)T.n - 2;
   const double ulast = last > 0 ? (double)last : 0.0;
-  long long k[N];
+  int k[N];
   double2 v[N][8];
 #pragma unroll
   for (int i = 0; i < N; ++i)
-    k[i] = (long long)fmin(fmax((t[i] - T.t0) * T.inv_h, 0.0), ulast);
+    k[i] = (int)fmin(fmax((t[i] - T.t0) * T.inv_h, 0.0), ulast);
 #pragma unroll
-  for (int i = 0; i < N; ++i) load_track_record(T.rec + 16 * k[i], v[i]);
+  for (int i = 0; i < N; ++i)
+    load_track_record(T.rec + 16 * (unsigned)k[i], v[i]);
+  // Every record is consumed here, after all loads have issued: without
+  // this the compiler sank a record's loads into a branch on the end
+  // tests (a select between a track end and a loaded value reads as a
+  // branch-worthy load), serialising that lookup behind the others.
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      asm volatile("" : "+v"(v[i][j].x));
+      asm volatile("" : "+v"(v[i][j].y));
+    }
   bool fix = false;
   bool bad[N];
 #pragma unroll
@@ -914,11 +999,13 @@ struct EvalMilkyWayTracked {
   static constexpr bool kInlineFirstStep = false;
   static constexpr bool kTrackPrefetch = true;
   static constexpr int kWavesPerSimd = 1;  // occupancy target (0: none)
+  // the track and the Plummer row (G m, r_s^2) are read from the by-value
+  // copies in P (P.trk, P.kp), not through P.tracks and P.comps
   template <int N>
   __device__ __forceinline__ static void track_n(const DPot& P,
                                                  const double (&t)[N],
                                                  double (&d)[N][3]) {
-    track_pos_n<N>(P.tracks[P.comps[4].track], t, d);
+    track_pos_n<N>(P.trk, t, d);
   }
   __device__ __forceinline__ static void acc_at(const DPot& P, double x,
                                                 double y, double z, double t,
@@ -926,10 +1013,9 @@ struct EvalMilkyWayTracked {
                                                 double& ax, double& ay,
                                                 double& az) {
     EvalMilkyWay::acc(P, x, y, z, t, ax, ay, az);
-    const DComp& c = P.comps[4];
     const double X = x - dx, Y = y - dy, Z = z - dz;
-    const double id = rsqrt_fast(X * X + Y * Y + Z * Z + c.p[1]);
-    const double g = c.p[0] * id * id * id;
+    const double id = rsqrt_fast(X * X + Y * Y + Z * Z + P.kp[1]);
+    const double g = P.kp[0] * id * id * id;
     ax -= g * X;
     ay -= g * Y;
     az -= g * Z;
@@ -938,8 +1024,73 @@ struct EvalMilkyWayTracked {
                                              double z, double t, double& ax,
                                              double& ay, double& az) {
     double dx, dy, dz;
-    track_pos(P.tracks[P.comps[4].track], t, dx, dy, dz);
+    track_pos(P.trk, t, dx, dy, dz);
     acc_at(P, x, y, z, t, dx, dy, dz, ax, ay, az);
+  }
+};
+
+// Dynamic LDS of the kernels whose evaluator stages its track there.
+extern __shared__ double2 gx_lds_track[];
+
+// The mock-stream field with the progenitor's track given as the
+// progenitor's own dense solution (a kTrackPieces track: one polynomial
+// piece per accepted step of its integration).  The launch stages the
+// pieces and their lookup cells in LDS (stage), and every stage-time lookup
+// reads LDS only: the cell of t, then the piece it names, then seven
+// coefficients per coordinate.  A few hundred pieces cover a progenitor's
+// orbit over Gyr, where a knot track fine enough for its cubics to match
+// the dense output holds tens of thousands of records that every lookup
+// fetches from device memory.  Field and end-holding as in
+// EvalMilkyWayTracked.
+struct EvalMilkyWayPieces : EvalMilkyWayTracked {
+  static constexpr bool kLdsTrack = true;
+  // every lane of the workgroup copies; the caller synchronises
+  __device__ __forceinline__ static void stage(const DPot& P) {
+    const double2* src = reinterpret_cast<const double2*>(P.trk.rec);
+    const int m = (int)(P.trk.lds_doubles / 2);
+    for (int i = threadIdx.x; i < m; i += blockDim.x) gx_lds_track[i] = src[i];
+  }
+  template <int N>
+  __device__ __forceinline__ static void track_n(const DPot& P,
+                                                 const double (&t)[N],
+                                                 double (&d)[N][3]) {
+    const DTrack& T = P.trk;
+    const double2* pc = gx_lds_track;
+    const double2* cl = gx_lds_track + (kPieceDoubles / 2) * (int)T.n;
+    const double cmax = (double)(T.ncell - 1);
+    // Times outside [t0, t1] are clamped to it: the first piece at t0 and
+    // the last at t1 give q0 and q1 bit for bit (the host computed those
+    // with the same Horner steps), so the held ends need no select -- a
+    // select between them and a looked-up value compiled to a branch
+    // around each lookup's reads, serialising the lookups.
+    double tc[N];
+    int k[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      tc[i] = fmin(fmax(t[i], T.t0), T.t1);
+      const int c = (int)fmin((tc[i] - T.t0) * T.inv_h, cmax);
+      const double2 e0 = cl[2 * c], e1 = cl[2 * c + 1];
+      k[i] = (int)e0.x + (tc[i] >= e0.y ? 1 : 0) + (tc[i] >= e1.x ? 1 : 0);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double2* r = pc + (kPieceDoubles / 2) * k[i];
+      double c[kPieceDoubles];
+#pragma unroll
+      for (int j = 0; j < kPieceDoubles / 2; ++j) {
+        const double2 v = r[j];
+        c[2 * j] = v.x;
+        c[2 * j + 1] = v.y;
+      }
+      const double th = (tc[i] - c[0]) * c[1];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        double v = c[2 + 7 * m + 6];
+#pragma unroll
+        for (int e = 5; e >= 0; --e) v = fma(v, th, c[2 + 7 * m + e]);
+        d[i][m] = v;
+      }
+    }
   }
 };
 

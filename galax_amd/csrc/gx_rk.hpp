@@ -23,24 +23,82 @@ template <class E>
 struct track_prefetch<E, std::void_t<decltype(E::kTrackPrefetch)>>
     : std::bool_constant<E::kTrackPrefetch> {};
 
+// The first stage j <= i whose node c_j equals stage i's (i itself when its
+// node is new).  Stage times tb + c_i h of equal nodes are the same double,
+// so their track positions are looked up once; the end stage (SF-1) sits at
+// te, which is computed apart from the nodes, and is always its own.
+template <class M>
+constexpr int same_node(int i) {
+  if (i == M::SF - 1) return i;
+  for (int j = 1; j < i; ++j)
+    if (M::kf.c[j] == M::kf.c[i]) return j;
+  return i;
+}
+
+// Stages i0 + 1 .. i0 + N as one batch of lookups: the U stages with a new
+// node (uniq), and for each stage its slot among them, or -1 - (j - 1) to
+// copy the position of the earlier stage j with the same node.
+template <class M, int i0, int N>
+struct NodeBatch {
+  struct Map {
+    int uniq[N];
+    int slot[N];
+    int U;
+  };
+  static constexpr Map make() {
+    Map m{};
+    for (int j = 0; j < N; ++j) {
+      const int i = i0 + 1 + j, s = same_node<M>(i);
+      if (s == i) {
+        m.uniq[m.U] = i;
+        m.slot[j] = m.U++;
+      } else {
+        m.slot[j] = -1 - (s - 1);
+      }
+    }
+    return m;
+  }
+  static constexpr Map map = make();
+  static constexpr int U = map.U;
+};
+
+// The first NA stages (1 .. NA) hold half of the step's distinct stage times
+// (rounded up): the split of the lookups into two batches of loads.
+template <class M>
+constexpr int half_split() {
+  int total = 0;
+  for (int i = 1; i < M::SF; ++i) total += same_node<M>(i) == i;
+  int seen = 0;
+  for (int i = 1; i < M::SF; ++i) {
+    seen += same_node<M>(i) == i;
+    if (2 * seen >= total) return i;
+  }
+  return M::SF - 1;
+}
+
 // Track positions at stage times i0 + 1 .. i0 + N of a step (the end stage
-// at te), into d[i0 .. i0 + N - 1].
+// at te), into d[i0 .. i0 + N - 1]; a stage whose node repeats an earlier
+// stage's copies that stage's position (which an earlier batch, or this
+// one, has filled).
 template <class M, class E, int i0, int N>
 __device__ __forceinline__ void stage_tracks(const DPot& P, double dir,
                                              double tb, double h, double te,
                                              double (&d)[M::SF - 1][3]) {
-  double tt[N], dd[N][3];
+  using B = NodeBatch<M, i0, N>;
+  constexpr int U = B::U;
+  double tt[U], dd[U][3];
 #pragma unroll
-  for (int j = 0; j < N; ++j) {
-    const int i = i0 + 1 + j;
-    tt[j] = dir * ((i == M::SF - 1) ? te : fma(M::kf.c[i], h, tb));
+  for (int u = 0; u < U; ++u) {
+    const int i = B::map.uniq[u];
+    tt[u] = dir * ((i == M::SF - 1) ? te : fma(M::kf.c[i], h, tb));
   }
-  E::template track_n<N>(P, tt, dd);
+  E::template track_n<U>(P, tt, dd);
 #pragma unroll
   for (int j = 0; j < N; ++j) {
-    d[i0 + j][0] = dd[j][0];
-    d[i0 + j][1] = dd[j][1];
-    d[i0 + j][2] = dd[j][2];
+    const int sl = B::map.slot[j];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      d[i0 + j][m] = sl >= 0 ? dd[sl >= 0 ? sl : 0][m] : d[sl >= 0 ? 0 : -1 - sl][m];
   }
 }
 
@@ -80,7 +138,7 @@ __device__ __forceinline__ void rk_step(const DPot& P, double dir, double tb,
   static_assert(!(kPre && E::kLarge), "track prefetch needs unrolled stages");
   double trk[kPre ? SF - 1 : 1][3];
   if constexpr (kPre) {
-    constexpr int NA = SF / 2;
+    constexpr int NA = half_split<M>();
     stage_tracks<M, E, 0, NA>(P, dir, tb, h, te, trk);
     stage_tracks<M, E, NA, SF - 1 - NA>(P, dir, tb, h, te, trk);
   }

@@ -19,8 +19,10 @@ threefry `jr.normal`; here they come from NumPy's PCG64 seeded by `rng`
 following the same distributions.
 
 With `progenitor_potential`, the particles also feel the progenitor: that
-potential is translated along the progenitor's own integrated track (cubic
-Hermite interpolation of its saved positions and velocities).
+potential is translated along the progenitor's own dense solution (the
+method's dense output over each of its accepted steps, the polynomial its
+saves come from); an orbit without one (computed with dense=False) gives a
+cubic Hermite track through its saved positions and velocities instead.
 
 `MockStreamGenerator.run` samples the release on the device (the King radius
 from the device Hessian, Fardal's formula in torch); `FardalStreamDF.sample`
@@ -214,12 +216,16 @@ class MockStreamGenerator:
 
     def stream_potential(self, prog_o):
         """The field the particles move in: the host, plus the progenitor's
-        own potential riding its track when one is given."""
+        own potential riding its track when one is given (the progenitor's
+        dense solution when `prog_o` carries one, else its saved states)."""
         if self.progenitor_potential is None:
             return self.potential
-        track = TimeDependentTranslationParameter(
-            np.asarray(prog_o.t), np.asarray(prog_o.q).reshape(-1, 3), np.asarray(prog_o.p).reshape(-1, 3)
-        )
+        if getattr(prog_o, "interpolant", None) is not None:
+            track = TimeDependentTranslationParameter.from_dense(prog_o)
+        else:
+            track = TimeDependentTranslationParameter(
+                np.asarray(prog_o.t), np.asarray(prog_o.q).reshape(-1, 3), np.asarray(prog_o.p).reshape(-1, 3)
+            )
         return CompositePotential(
             host=self.potential, progenitor=TranslatedPotential(self.progenitor_potential, track)
         )
@@ -234,7 +240,8 @@ class MockStreamGenerator:
         else:
             w, _, t = parse_w0(prog_w0)
             w0 = PhaseSpaceCoordinate(q=w[0, :3], p=w[0, 3:], t=ts[0] if t is None else t)
-        prog_o = evaluate_orbit(self.potential, w0, ts, integrator=self.progenitor_integrator)
+        prog_o = evaluate_orbit(self.potential, w0, ts, integrator=self.progenitor_integrator,
+                                dense=self.progenitor_potential is not None)
         prog_o.q = np.asarray(prog_o.q).reshape(-1, 3)
         prog_o.p = np.asarray(prog_o.p).reshape(-1, 3)
         t_f = ts[-1] + 1e-3

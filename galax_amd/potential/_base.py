@@ -23,7 +23,7 @@ import numpy as np
 from .. import _abi, _lib
 from ..units import G, Q, ustrip
 
-__all__ = ["AbstractPotential", "AbstractSinglePotential", "Track"]
+__all__ = ["AbstractPotential", "AbstractSinglePotential", "DenseTrack", "Track"]
 
 
 @dataclass(frozen=True)
@@ -34,6 +34,38 @@ class Track:
     t: np.ndarray
     q: np.ndarray
     v: np.ndarray
+
+
+@dataclass(frozen=True)
+class DenseTrack:
+    """Centre trajectory given as the dense solution of one integrated
+    orbit: its accepted-step records rec[n, GX_DENSE_RECORD] in integration
+    order, written by method `method` (GX_DOPRI8 or GX_DOPRI5); the position
+    at t is the method's dense output over the step holding t."""
+
+    rec: np.ndarray
+    method: int
+
+
+def fill_track(dst, tr) -> list:
+    """Fill a gx_track from a Track or a DenseTrack; returns the arrays it
+    points into, which must outlive the call that reads it."""
+    if isinstance(tr, DenseTrack):
+        rec = np.ascontiguousarray(tr.rec, dtype=np.float64)
+        if rec.ndim != 2 or rec.shape[1] != _abi.GX_DENSE_RECORD or rec.shape[0] < 1:
+            raise ValueError(f"dense track records must have shape (n >= 1, {_abi.GX_DENSE_RECORD})")
+        dst.n = rec.shape[0]
+        dst.rec = rec.ctypes.data
+        dst.method = int(tr.method)
+        return [rec]
+    tt = np.ascontiguousarray(tr.t, dtype=np.float64)
+    qq = np.ascontiguousarray(tr.q, dtype=np.float64)
+    vv = np.ascontiguousarray(tr.v, dtype=np.float64)
+    dst.n = tt.shape[0]
+    dst.t = tt.ctypes.data
+    dst.q = qq.ctypes.data
+    dst.v = vv.ctypes.data
+    return [tt, qq, vv]
 
 
 def _destroy_handles(handles: dict) -> None:
@@ -83,14 +115,7 @@ class AbstractPotential:
         keep = []
         tarr = (_abi.GxTrack * max(len(tracks), 1))()
         for i, tr in enumerate(tracks):
-            tt = np.ascontiguousarray(tr.t, dtype=np.float64)
-            qq = np.ascontiguousarray(tr.q, dtype=np.float64)
-            vv = np.ascontiguousarray(tr.v, dtype=np.float64)
-            keep += [tt, qq, vv]
-            tarr[i].n = tt.shape[0]
-            tarr[i].t = tt.ctypes.data
-            tarr[i].q = qq.ctypes.data
-            tarr[i].v = vv.ctypes.data
+            keep += fill_track(tarr[i], tr)
         h = ctypes.c_void_p()
         rc = _lib.lib().gx_potential_create(
             ctypes.addressof(carr), len(comps), ctypes.addressof(tarr), len(tracks), ctypes.byref(h)

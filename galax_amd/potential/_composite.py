@@ -16,7 +16,7 @@ import numpy as np
 
 from .. import _abi
 from ..units import Q, ustrip
-from ._base import AbstractPotential, Track
+from ._base import AbstractPotential, DenseTrack, Track
 from ._builtin import (
     HernquistPotential,
     LMJ09LogarithmicPotential,
@@ -239,10 +239,11 @@ class TimeDependentTranslationParameter:
     """Tabulated centre track.  Given positions only, the track is the
     not-a-knot cubic spline through them, as the reference's interpax
     CubicSpline (xfm/translate.py:185-199); given velocities as well, it is
-    the cubic Hermite interpolant of positions and velocities at the knots
-    (what the mock-stream generator passes: the progenitor's own states).
-    Outside the knot range the track holds its end positions, where the
-    reference's spline (extrapolate=False) gives NaN."""
+    the cubic Hermite interpolant of positions and velocities at the knots.
+    `from_dense` takes an integrated orbit's own dense solution instead
+    (what the mock-stream generator passes for the progenitor).  Outside the
+    covered range the track holds its end positions, where the reference's
+    spline (extrapolate=False) gives NaN."""
 
     def __init__(self, t, q, v=None):
         t = np.asarray(ustrip(t, "Myr") if isinstance(t, Q) else t, dtype=np.float64)
@@ -258,6 +259,26 @@ class TimeDependentTranslationParameter:
     @classmethod
     def from_(cls, t, q, v=None):
         return cls(t, q, v)
+
+    @classmethod
+    def from_dense(cls, solution, i: int = 0):
+        """The dense solution of orbit i of a solve (a DenseSolution, or an
+        Orbit computed with dense=True) as the track: between the solve's
+        accepted steps the position is the method's dense output, the same
+        polynomial the solve's saves and Orbit.interpolant come from (no
+        knots, no interpolation error beyond the solve's own).  Outside the
+        integrated interval the end positions are held."""
+        dense = getattr(getattr(solution, "interpolant", None), "dense", solution)
+        if not hasattr(dense, "rec") or not hasattr(dense, "nrec"):
+            raise TypeError("from_dense needs a DenseSolution or an Orbit computed with dense=True")
+        nr = int(dense.nrec[i])
+        if nr < 1:
+            raise ValueError(f"orbit {i} of the dense solution has no accepted steps")
+        rec = dense.rec[i, :nr]
+        rec = rec.cpu().numpy() if hasattr(rec, "cpu") else np.asarray(rec)
+        obj = cls.__new__(cls)
+        obj.track = DenseTrack(rec=np.ascontiguousarray(rec, dtype=np.float64), method=int(dense.method))
+        return obj
 
     @classmethod
     def from_callable(cls, fn, t):

@@ -139,14 +139,27 @@ typedef struct gx_component {
   double f[6];      /* frame parameters (see gx_frame) */
 } gx_component;
 
-/* A tabulated centre trajectory: n knots, strictly increasing t[n],
- * positions q[n*3] and velocities v[n*3] (row-major).  Outside [t0, t_{n-1}]
- * the end knot is held (constant position). */
+/* A centre trajectory, in one of two forms.
+ * Knots (rec == NULL): n knots (1 <= n <= INT32_MAX), strictly increasing
+ *   t[n], positions q[n*3] and velocities v[n*3] (row-major), interpolated
+ *   by cubic Hermite polynomials.
+ * Dense solution (rec != NULL): the dense output of one integrated orbit,
+ *   n accepted-step records (1 <= n <= INT32_MAX) of GX_DENSE_RECORD
+ *   doubles in integration order, as gx_integrate_dense writes them for one
+ *   orbit, from method `method` (gx_method); consecutive records must be
+ *   contiguous in time (each starts where the previous one ends).  The
+ *   position at t is the method's dense output over the step holding t
+ *   (diffrax's DenseInterpolation, which Orbit.interpolant evaluates); t,
+ *   q and v are ignored.
+ * Outside the covered interval the end position is held. */
 typedef struct gx_track {
   int64_t n;
   const double* t;
   const double* q;
   const double* v;
+  const double* rec;
+  int32_t method;
+  int32_t reserved; /* 0 */
 } gx_track;
 
 typedef struct gx_pot gx_pot; /* opaque handle; lives on one device */

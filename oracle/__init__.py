@@ -71,15 +71,10 @@ class _Table:
         self.n = len(comps)
         self.keep = []
         self.tracks = (_abi.GxTrack * max(len(tracks), 1))()
+        from galax_amd.potential._base import fill_track
+
         for i, tr in enumerate(tracks):
-            t = np.ascontiguousarray(tr.t, dtype=np.float64)
-            q = np.ascontiguousarray(tr.q, dtype=np.float64)
-            v = np.ascontiguousarray(tr.v, dtype=np.float64)
-            self.keep += [t, q, v]
-            self.tracks[i].n = t.shape[0]
-            self.tracks[i].t = t.ctypes.data
-            self.tracks[i].q = q.ctypes.data
-            self.tracks[i].v = v.ctypes.data
+            self.keep += fill_track(self.tracks[i], tr)
         self.ntracks = len(tracks)
 
     def args(self):

@@ -627,10 +627,21 @@ struct Track {
   const double* t = nullptr;
   const double* q = nullptr;
   const double* v = nullptr;
+  // a dense solution instead (rec != nullptr): n step records of
+  // GX_DENSE_RECORD doubles in integration order, written by `method`
+  const double* rec = nullptr;
+  int method = 0;
 };
 
-// Cubic Hermite interpolant of a tabulated trajectory (end knots held).
+void dense_track_pos(const Track& tr, double t, double out[3]);
+
+// Cubic Hermite interpolant of a tabulated trajectory (end knots held), or
+// a dense solution's own dense output.
 void track_pos(const Track& tr, double t, double out[3]) {
+  if (tr.rec) {
+    dense_track_pos(tr, t, out);
+    return;
+  }
   if (tr.n <= 0) {
     out[0] = out[1] = out[2] = 0;
     return;
@@ -1092,6 +1103,81 @@ void dense_eval(const Tableau& T, const double* y0, const double* y1,
   }
 }
 
+// Position of a dense-solution track at t (galax's Orbit interpolant over
+// diffrax's DenseInterpolation, orbit/interp.py:42-145): the record whose
+// step holds t, then the method's first-order dense output (dense_eval)
+// over it, with the increments rebuilt from the record: stage velocities
+// v_i = v0 + h sum_j a_ij g_j, increments k_i = h (v_i, g_i), and for
+// Dopri8's extra slot k = h (v1, g_end), v1 = v0 + h sum_j b_j g_j.  Outside
+// the integrated interval the end position is held.
+void record_pos(const Tableau& T, const double* r, double th, double out[3]) {
+  const int s = T.s, ns = dense_slots(T);
+  const double h = r[1];
+  const double* g = r + 8;
+  double y0[6], y1[6];
+  double k[14 * 6] = {};
+  for (int m = 0; m < 3; ++m) {
+    y0[m] = r[2 + m];
+    y0[3 + m] = r[5 + m];
+  }
+  double q1[3] = {y0[0], y0[1], y0[2]}, v1[3] = {y0[3], y0[4], y0[5]};
+  for (int i = 0; i < s; ++i) {
+    for (int m = 0; m < 3; ++m) {
+      double vi = y0[3 + m];
+      for (int j = 0; j < i; ++j) vi += h * T.a[i][j] * g[3 * j + m];
+      k[(size_t)i * 6 + m] = h * vi;
+      k[(size_t)i * 6 + 3 + m] = h * g[3 * i + m];
+      q1[m] += T.b[i] * h * vi;
+      v1[m] += T.b[i] * h * g[3 * i + m];
+    }
+  }
+  for (int m = 0; m < 3; ++m) {
+    y1[m] = q1[m];
+    y1[3 + m] = v1[m];
+  }
+  if (ns > s)
+    for (int m = 0; m < 3; ++m) {
+      k[(size_t)s * 6 + m] = h * v1[m];
+      k[(size_t)s * 6 + 3 + m] = h * g[3 * s + m];
+    }
+  double o[6];
+  dense_eval(T, y0, y1, k, 1, th, o);
+  for (int m = 0; m < 3; ++m) out[m] = o[m];
+}
+
+void dense_track_pos(const Track& tr, double t, double out[3]) {
+  const Tableau& T = tableau(tr.method);
+  const int64_t n = tr.n;
+  const bool fwd = tr.rec[1] > 0.0;
+  // records by increasing time: j -> record idx(j); left end of each
+  auto idx = [&](int64_t j) { return fwd ? j : n - 1 - j; };
+  auto left = [&](int64_t j) {
+    const double* r = tr.rec + GX_DENSE_RECORD * idx(j);
+    return fwd ? r[0] : r[0] + r[1];
+  };
+  const double* first = tr.rec + GX_DENSE_RECORD * idx(0);
+  const double* last = tr.rec + GX_DENSE_RECORD * idx(n - 1);
+  const double t_lo = left(0), t_hi = fwd ? last[0] + last[1] : last[0];
+  if (t <= t_lo) {
+    record_pos(T, first, fwd ? 0.0 : 1.0, out);
+    return;
+  }
+  if (t >= t_hi) {
+    record_pos(T, last, fwd ? 1.0 : 0.0, out);
+    return;
+  }
+  int64_t lo = 0, hi = n - 1;  // last j with left(j) <= t
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) / 2;
+    if (left(mid) <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const double* r = tr.rec + GX_DENSE_RECORD * idx(lo);
+  record_pos(T, r, (t - r[0]) / r[1], out);
+}
+
 double rms(const double* x, int64_t n) {
   double s = 0;
   for (int64_t i = 0; i < n; ++i) s += x[i] * x[i];
@@ -1295,6 +1381,8 @@ Pot make_pot(const gx_component* comps, int ncomp, const gx_track* tracks,
     t.t = tracks[i].t;
     t.q = tracks[i].q;
     t.v = tracks[i].v;
+    t.rec = tracks[i].rec;
+    t.method = tracks[i].method;
     P.tracks.push_back(t);
   }
   return P;

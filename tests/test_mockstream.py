@@ -66,7 +66,8 @@ def test_tidal_radius_on_gpu_matches_closed_form():
 @pytest.mark.gpu
 def test_stream_release_integration_parity():
     """A 2x300-particle stream over 1 Gyr with the progenitor's own Plummer
-    potential on its track: the single-launch GPU release integration agrees
+    potential riding the progenitor's dense solution: the single-launch GPU
+    release integration (MockStreamGenerator.run, track staged in LDS) agrees
     with the oracle integrating the same ICs in the same field."""
     mw = gp.MilkyWayPotential()
     prog = gp.PlummerPotential(m_tot=1e5, r_s=0.01)
@@ -78,12 +79,13 @@ def test_stream_release_integration_parity():
     assert np.all(stream.status == 0)
     assert stream["lead"].q.shape == (300, 3)
     # rebuild the same ICs and field, integrate with the oracle
-    prog_o = gd.evaluate_orbit(mw, w0, ts, integrator=gen.progenitor_integrator)
+    prog_o = gd.evaluate_orbit(mw, w0, ts, integrator=gen.progenitor_integrator, dense=True)
     prog_o.q = prog_o.q.reshape(-1, 3)
     prog_o.p = prog_o.p.reshape(-1, 3)
     ics = gd.FardalStreamDF().sample(7, mw, prog_o, 1e5)
     w_all = np.concatenate([ics["lead"].w(), ics["trail"].w()])
     field = gen.stream_potential(prog_o)
+    assert type(field["progenitor"].translation.track).__name__ == "DenseTrack"
     ref, st, _ns, _ = oracle.integrate(field, gen.stream_integrator.dynamics_solver, w_all,
                                    np.concatenate([ts, ts]), ts[-1] + 1e-3, max_steps=None)
     got = np.concatenate([stream["lead"].w(), stream["trail"].w()])
@@ -92,6 +94,35 @@ def test_stream_release_integration_parity():
     floor = noise_floor(field, gen.stream_integrator.dynamics_solver, w_all, np.concatenate([ts, ts]),
                         ts[-1] + 1e-3, ref, _ns, max_steps=None)
     check_adaptive(got[:, None, :], ref, stream.nsteps, _ns, floor)
+
+
+@pytest.mark.gpu
+def test_stream_knot_track_integration_parity():
+    """The same stream field with the progenitor's track given as knots (its
+    saved positions and velocities, cubic Hermite), what stream_potential
+    builds from an orbit without a dense solution: GPU against the oracle."""
+    mw = gp.MilkyWayPotential()
+    prog = gp.PlummerPotential(m_tot=1e5, r_s=0.01)
+    gen = gd.MockStreamGenerator(gd.FardalStreamDF(), mw, progenitor_potential=prog)
+    ts = np.linspace(-1000.0, 0.0, 300)
+    w0 = gd.PhaseSpaceCoordinate(q=np.array([10.0, 0.0, 2.0]), p=np.array([0.0, 0.2, 0.05]), t=-1000.0)
+    prog_o = gd.evaluate_orbit(mw, w0, ts, integrator=gen.progenitor_integrator)
+    prog_o.q = prog_o.q.reshape(-1, 3)
+    prog_o.p = prog_o.p.reshape(-1, 3)
+    field = gen.stream_potential(prog_o)
+    assert type(field["progenitor"].translation.track).__name__ == "Track"
+    ics = gd.FardalStreamDF().sample(7, mw, prog_o, 1e5)
+    w_all = np.concatenate([ics["lead"].w(), ics["trail"].w()])
+    t0 = np.concatenate([ts, ts])
+    solver = gen.stream_integrator.dynamics_solver
+    print("[case] stream 600 particles, knot track", flush=True)
+    res = gd.integrate_orbits(field, w_all, t0, np.full(t0.shape, ts[-1] + 1e-3), solver=solver)
+    got = res.out.cpu().numpy()
+    ref, st, _ns, _ = oracle.integrate(field, solver, w_all, t0, ts[-1] + 1e-3, max_steps=None)
+    from test_gpu_parity import check_adaptive, noise_floor
+
+    floor = noise_floor(field, solver, w_all, t0, ts[-1] + 1e-3, ref, _ns, max_steps=None)
+    check_adaptive(got, ref, res.nsteps.cpu().numpy(), _ns, floor)
 
 
 @pytest.mark.gpu
