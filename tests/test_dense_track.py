@@ -129,12 +129,15 @@ def _satellite_orbits(prog_o, ts, every):
 
 
 @pytest.mark.gpu
-def test_dense_track_field_matches_oracle():
+@pytest.mark.parametrize("backward", [False, True])
+def test_dense_track_field_matches_oracle(backward):
     """gx_acceleration on the Milky Way plus a Plummer sphere riding a
-    progenitor's dense solution, at times across and beyond the solve,
-    against the oracle's own dense output of the same records."""
+    progenitor's dense solution (integrated forward, or backward in time),
+    at times across and beyond the solve, against the oracle's own dense
+    output of the same records."""
     ts = np.linspace(-2000.0, 0.0, 400)
-    mw, prog_o = _progenitor(ts)
+    mw, prog_o = _progenitor(ts[::-1].copy() if backward else ts)
+    assert (float(prog_o.interpolant.dense.rec[0, 0, 1]) < 0) == backward
     field = _stream_field(mw, prog_o)
     rng = np.random.default_rng(5)
     t = np.concatenate([rng.uniform(-2000.0, 0.0, 3000), [-2500.0, -2000.0, 0.0, 1.0]])
@@ -142,7 +145,7 @@ def test_dense_track_field_matches_oracle():
     centre[t < -2000.0] = centre[t == -2000.0][0]
     centre[t > 0.0] = centre[t == 0.0][0]
     x = centre + rng.normal(0.0, 0.05, centre.shape)
-    print("[case] dense track field, 3004 points", flush=True)
+    print(f"[case] dense track field, 3004 points, backward={backward}", flush=True)
     got = np.asarray(field.acceleration(x, t))
     want = oracle.evaluate(field, "acceleration", x, t)
     err = np.linalg.norm(got - want, axis=-1) / np.linalg.norm(want, axis=-1)
@@ -150,28 +153,34 @@ def test_dense_track_field_matches_oracle():
 
 
 @pytest.mark.gpu
-def test_lds_track_integration_matches_oracle_fixed_step():
+@pytest.mark.parametrize("backward", [False, True])
+def test_lds_track_integration_matches_oracle_fixed_step(backward):
     """The mock-stream integrator with the track staged in LDS (a few dozen
     pieces) at a fixed step: every stage's track lookup against the oracle.
     The orbits circle a 1e9 Msun progenitor, whose pull changes them by more
     than 1e-3 over the run (checked), so a misplaced track would show far
     above the 1e-11 bound; the differences that remain are round-off grown
-    by the encounters (1e-14 to 2e-12)."""
+    by the encounters (1e-14 to 2e-12).  Backward: the progenitor and the
+    orbits integrated from the present into the past (pieces of negative
+    step, staged in reverse)."""
     ts = np.linspace(-300.0, 0.0, 300)
+    if backward:
+        ts = ts[::-1].copy()
     mw, prog_o = _progenitor(ts)
     field = _stream_field(mw, prog_o, m=1e9, b=0.3)
     nrec = int(prog_o.interpolant.dense.nrec[0])
     assert 8 <= nrec <= 512, nrec
     w0, t0 = _satellite_orbits(prog_o, ts, 2)
+    t_end = float(ts[-1])
     solver = gd.OrbitSolver(gd.Dopri8(), stepsize_controller=gd.ConstantStepSize(), max_steps=None)
-    print(f"[case] LDS track fixed step, {w0.shape[0]} orbits, {nrec} pieces", flush=True)
-    res = gd.integrate_orbits(field, w0, t0, np.full(t0.shape, 0.0), solver=solver, dt0=0.5)
+    print(f"[case] LDS track fixed step, {w0.shape[0]} orbits, {nrec} pieces, backward={backward}", flush=True)
+    res = gd.integrate_orbits(field, w0, t0, np.full(t0.shape, t_end), solver=solver, dt0=0.5)
     got = res.out[:, 0, :].cpu().numpy()
-    ref, st, _, _ = oracle.integrate(field, solver, w0, t0, 0.0, dt0=0.5, max_steps=None)
+    ref, st, _, _ = oracle.integrate(field, solver, w0, t0, t_end, dt0=0.5, max_steps=None)
     assert np.all(res.status.cpu().numpy() == 0) and np.all(st == 0)
     rel = np.linalg.norm(got - ref[:, -1], axis=-1) / np.linalg.norm(ref[:, -1], axis=-1)
     assert rel.max() < 1e-11, rel.max()
-    free, _, _, _ = oracle.integrate(mw, solver, w0, t0, 0.0, dt0=0.5, max_steps=None)
+    free, _, _, _ = oracle.integrate(mw, solver, w0, t0, t_end, dt0=0.5, max_steps=None)
     pull = np.linalg.norm(free[:, -1] - ref[:, -1], axis=-1) / np.linalg.norm(ref[:, -1], axis=-1)
     assert np.median(pull) > 1e-3, np.median(pull)
 
