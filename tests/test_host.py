@@ -242,13 +242,15 @@ def test_integrator_kernels_keep_their_occupancy(tmp_path):
     (two waves per SIMD; above it a SIMD holds one wave of them and the
     throughput halves): a guard against additions to the field code that
     the compiler inlines into every kernel.  The tracked-Plummer integrator
-    runs one wave per SIMD by design (its batched track loads take the
-    registers) and must stay within that wave's 512 without scratch memory
-    (its few spills go to accumulation registers)."""
+    on a knot track runs one wave per SIMD by design (its batched track
+    loads take the registers) and must stay within that wave's 512 without
+    scratch memory (its few spills go to accumulation registers).  On an
+    LDS-staged dense track it fits 256, so that two workgroups share a CU
+    and a 100,000-particle release starts every particle at once."""
     res = _kernel_resources(tmp_path)
     # key: (VGPR limit, VGPR spill limit or None, scratch memory allowed)
     want = {"EvalMW": (256, 0, False), "EvalTableILb0E": (256, None, True),
-            "EvalMilkyWayTracked": (512, None, False)}
+            "EvalMilkyWayTracked": (512, None, False), "EvalMilkyWayPieces": (256, None, False)}
     for key, (max_vgpr, max_spill, scratch_ok) in want.items():
         ks = {n: v for n, v in res.items() if "k_integrate" in n and "Dopri8" in n and key in n}
         assert ks, key
