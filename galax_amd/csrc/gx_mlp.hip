@@ -61,9 +61,16 @@ struct GemmArgs {
   long long ldzin;
 };
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative in FP32 with the hardware reciprocal (1 ulp): the
+// results are rounded to BF16 (8 bits), and a correctly rounded division
+// took ten instructions per activation.  exp(-z) = +inf for very negative
+// z gives 1/inf = 0, so silu -> -0 and dsilu -> 0 there, as in the limit.
+__device__ __forceinline__ float sigmoid(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float dsilu(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -197,10 +204,11 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
 // hidden-to-hidden layer runs on the matrix cores with the same operand
 // layout as k_gemm, the layer's BF16 weights read from L2 (each layer's
 // 128 KB is shared by every block).  Each wave owns 64 output columns of a
-// layer: 4 x 4 MFMA tiles.  The input layer (8 padded inputs) and the output
-// layer (a few columns) are short dot products in FP32.  Rounding as the
-// layer-by-layer path: FP32 accumulation, bias + SiLU in FP32, BF16
-// activations.
+// layer: 4 x 4 MFMA tiles.  The input layer (8 padded inputs) is a short
+// dot product in FP32 over the block's inputs staged in LDS; the output
+// layer (at most 16 columns, zero-padded) is one MFMA tile column per wave.
+// Rounding as the layer-by-layer path: FP32 accumulation, bias + SiLU in
+// FP32, BF16 activations.
 constexpr int FH = 256;      // hidden width
 constexpr int FM = 64;       // points per block
 constexpr int FLD = FH + 8;  // padded LDS row (bf16 elements)
@@ -220,9 +228,17 @@ struct FusedArgs {
 
 __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs f) {
   __shared__ __bf16 act[2][FM][FLD];
+  __shared__ float xin[FM][8];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const long long m0 = (long long)blockIdx.x * FM;
+  // ---- the block's inputs, once (zero past n and past nin)
+  for (int t = tid; t < FM * 8; t += 256) {
+    const int r = t >> 3, k = t & 7;
+    const long long m = m0 + r;
+    xin[r][k] = (m < f.n && k < f.nin) ? (float)f.X[m * f.ldx + k] : 0.0f;
+  }
+  __syncthreads();
   // ---- input layer: thread tid owns hidden unit tid for all 64 points
   {
     const int c = tid;
@@ -232,13 +248,9 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs f) {
       w[k] = k < f.nin ? (float)f.W[0][c * f.nin + k] : 0.0f;
     const float bias = f.b[0][c];
     for (int r = 0; r < FM; ++r) {
-      const long long m = m0 + r;
       float z = 0.0f;
-      if (m < f.n) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-          if (k < f.nin) z = fmaf((float)f.X[m * f.ldx + k], w[k], z);
-      }
+      for (int k = 0; k < 8; ++k) z = fmaf(xin[r][k], w[k], z);
       act[0][r][c] = (__bf16)silu(z + bias);
     }
   }
@@ -284,20 +296,29 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs f) {
     cur ^= 1;
   }
   __syncthreads();
-  // ---- output layer: one (point, output) pair per thread
-  const __bf16* Wo = f.W[f.depth];
-  for (int t = tid; t < FM * f.nout; t += 256) {
-    const int r = t / f.nout, o = t - r * f.nout;
-    const long long m = m0 + r;
-    if (m >= f.n) continue;
-    float z = 0.0f;
-    for (int k = 0; k < FH; k += 8) {
-      const bf16x8 hv = *reinterpret_cast<const bf16x8*>(&act[cur][r][k]);
-      const bf16x8 wv = *reinterpret_cast<const bf16x8*>(Wo + o * FH + k);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) z = fmaf((float)hv[j], (float)wv[j], z);
+  // ---- output layer on the matrix cores: wave w takes points
+  // w*16 .. w*16+15 against the outputs padded to 16 columns (zero weights
+  // past nout)
+  {
+    const __bf16* Wo = f.W[f.depth];
+    const int n = lane & 15;
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+    for (int k0 = 0; k0 < FH; k0 += 32) {
+      const int kk = k0 + (lane >> 4) * 8;
+      const bf16x8 af = *reinterpret_cast<const bf16x8*>(&act[cur][wave * 16 + n][kk]);
+      bf16x8 bw{};
+      if (n < f.nout) bw = *reinterpret_cast<const bf16x8*>(Wo + n * FH + kk);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bw, acc, 0, 0, 0);
     }
-    f.Y[m * f.ldy + o] = z + f.b[f.depth][o];
+    if (n < f.nout) {
+      const float bo = f.b[f.depth][n];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const long long m = m0 + wave * 16 + (lane >> 4) * 4 + j;
+        if (m < f.n) f.Y[m * f.ldy + n] = acc[j] + bo;
+      }
+    }
   }
 }
 
