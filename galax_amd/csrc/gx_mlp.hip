@@ -446,7 +446,11 @@ __global__ void k_sur_targets(const double* __restrict__ t, int t_stride,
 
 // The surrogate field: a = a_base(x) (exact FP64) + scale Rz(th) Y[i], or the
 // exact full field where the gate says so (outside the trained ball
-// |x| > r_max, or an orbit flagged by the periodic check).
+// |x| > r_max, or an orbit flagged by the periodic check).  EB and EF
+// evaluate the base and full fields: the compile-time evaluators when the
+// tables have their signatures (the Milky Way model; it plus a rotating
+// group), else the table loop.
+template <class EB, class EF>
 __global__ void k_sur_combine(DPot base, DPot full, const double* __restrict__ xyz,
                               const double* __restrict__ t, int t_stride, double dt,
                               long long n, const float* __restrict__ Y,
@@ -460,9 +464,9 @@ __global__ void k_sur_combine(DPot base, DPot full, const double* __restrict__ x
   const bool exact = (x * x + y * y + z * z > r_max * r_max) || (flag && flag[i]);
   double ax, ay, az;
   if (exact) {
-    accel<true>(full, x, y, z, ti, ax, ay, az);
+    EF::acc(full, x, y, z, ti, ax, ay, az);
   } else {
-    accel<false>(base, x, y, z, ti, ax, ay, az);
+    EB::acc(base, x, y, z, ti, ax, ay, az);
     double s, c;
     sincos(omega * ti, &s, &c);
     const double px = scale * (double)Y[i * ldy], py = scale * (double)Y[i * ldy + 1];
@@ -478,6 +482,7 @@ __global__ void k_sur_combine(DPot base, DPot full, const double* __restrict__ x
 // The periodic gate check: the surrogate field at the current positions
 // against the exact field; an orbit whose relative error exceeds `tol` is
 // flagged (sticky) and integrated with the exact field from then on.
+template <class EF>
 __global__ void k_sur_check(DPot full, const double* __restrict__ xyz,
                             const double* __restrict__ t, int t_stride, long long n,
                             const double* __restrict__ a_sur, double tol,
@@ -485,8 +490,7 @@ __global__ void k_sur_check(DPot full, const double* __restrict__ xyz,
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double ax, ay, az;
-  accel<true>(full, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], t[t_stride ? i : 0], ax, ay,
-              az);
+  EF::acc(full, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], t[t_stride ? i : 0], ax, ay, az);
   const double ex = a_sur[3 * i] - ax, ey = a_sur[3 * i + 1] - ay, ez = a_sur[3 * i + 2] - az;
   const double e = sqrt((ex * ex + ey * ey + ez * ez) / (ax * ax + ay * ay + az * az));
   if (err) err[i] = (float)e;
@@ -693,9 +697,15 @@ int gx_sur_combine(const gx_pot* base, const gx_pot* full, const double* xyz,
   if (!base->basic || base->device != full->device) return GX_EINVAL;
   if (n == 0) return GX_OK;
   if (!grid_ok(n)) return GX_EINVAL;
-  hipLaunchKernelGGL(k_sur_combine, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
-                     (hipStream_t)stream, base->dev, full->dev, xyz, t, t_stride, dt,
-                     (long long)n, Y, (long long)ldy, scale, omega, r_max, flag, acc);
+  const dim3 grid((unsigned)blocks_for(n, 256)), blk(256);
+  const bool mw = base->sig == kSigMilkyWay, rot = full->sig == kSigMilkyWayRotating;
+  const auto kern = mw ? (rot ? k_sur_combine<EvalMW, EvalMilkyWayRotating>
+                              : k_sur_combine<EvalMW, EvalTable<true>>)
+                       : (rot ? k_sur_combine<EvalTable<false>, EvalMilkyWayRotating>
+                              : k_sur_combine<EvalTable<false>, EvalTable<true>>);
+  hipLaunchKernelGGL(kern, grid, blk, 0, (hipStream_t)stream, base->dev, full->dev, xyz, t,
+                     t_stride, dt, (long long)n, Y, (long long)ldy, scale, omega, r_max, flag,
+                     acc);
   return check(hipGetLastError());
 }
 
@@ -705,7 +715,9 @@ int gx_sur_check(const gx_pot* full, const double* xyz, const double* t,
   if (!full || !xyz || !t || !a_sur || !flag || n < 0) return GX_EINVAL;
   if (n == 0) return GX_OK;
   if (!grid_ok(n)) return GX_EINVAL;
-  hipLaunchKernelGGL(k_sur_check, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
+  const auto kern = full->sig == kSigMilkyWayRotating ? k_sur_check<EvalMilkyWayRotating>
+                                                      : k_sur_check<EvalTable<true>>;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
                      (hipStream_t)stream, full->dev, xyz, t, t_stride, (long long)n, a_sur,
                      tol, flag, err);
   return check(hipGetLastError());
