@@ -32,6 +32,14 @@ namespace {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+typedef __attribute__((__vector_size__(4 * sizeof(unsigned)))) unsigned u32x4;
+
+// x, or zero where keep is false, by a bit mask: loads feeding it stay
+// unconditional (a select between a loaded value and zero compiled to a
+// branch around the load and a wait after it).
+__device__ __forceinline__ float keep_or_zero(float x, bool keep) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(unsigned, x) & (keep ? ~0u : 0u));
+}
 
 constexpr int BM = 64, BN = 64, BK = 32;
 constexpr int LDSK = BK + 8;  // padded row (bf16 elements) against bank conflicts
@@ -233,10 +241,14 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs f) {
   const int lane = tid & 63, wave = tid >> 6;
   const long long m0 = (long long)blockIdx.x * FM;
   // ---- the block's inputs, once (zero past n and past nin)
-  for (int t = tid; t < FM * 8; t += 256) {
+  static_assert(FM * 8 % 256 == 0, "input staging: whole rounds of the block");
+#pragma unroll
+  for (int j = 0; j < FM * 8 / 256; ++j) {
+    const int t = tid + 256 * j;
     const int r = t >> 3, k = t & 7;
     const long long m = m0 + r;
-    xin[r][k] = (m < f.n && k < f.nin) ? (float)f.X[m * f.ldx + k] : 0.0f;
+    const bool in = m < f.n && k < f.nin;  // else an in-range stand-in, masked
+    xin[r][k] = keep_or_zero((float)f.X[(m < f.n ? m : f.n - 1) * f.ldx + (k < f.nin ? k : 0)], in);
   }
   __syncthreads();
   // ---- input layer: thread tid owns hidden unit tid for all 64 points
@@ -245,7 +257,7 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs f) {
     float w[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-      w[k] = k < f.nin ? (float)f.W[0][c * f.nin + k] : 0.0f;
+      w[k] = keep_or_zero((float)f.W[0][c * f.nin + (k < f.nin ? k : 0)], k < f.nin);
     const float bias = f.b[0][c];
     for (int r = 0; r < FM; ++r) {
       float z = 0.0f;
@@ -302,14 +314,22 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs f) {
   {
     const __bf16* Wo = f.W[f.depth];
     const int n = lane & 15;
+    // all eight weight fragments in flight at once (row 0 stands in for the
+    // padding columns, whose fragments are then masked to zero)
+    const int nr = n < f.nout ? n : 0;
+    const unsigned mask = n < f.nout ? ~0u : 0u;
+    bf16x8 bw[FH / 32];
+#pragma unroll
+    for (int s = 0; s < FH / 32; ++s) {
+      const u32x4 v = *reinterpret_cast<const u32x4*>(Wo + nr * FH + s * 32 + (lane >> 4) * 8);
+      bw[s] = __builtin_bit_cast(bf16x8, v & mask);
+    }
     f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-    for (int k0 = 0; k0 < FH; k0 += 32) {
-      const int kk = k0 + (lane >> 4) * 8;
+#pragma unroll
+    for (int s = 0; s < FH / 32; ++s) {
+      const int kk = s * 32 + (lane >> 4) * 8;
       const bf16x8 af = *reinterpret_cast<const bf16x8*>(&act[cur][wave * 16 + n][kk]);
-      bf16x8 bw{};
-      if (n < f.nout) bw = *reinterpret_cast<const bf16x8*>(Wo + n * FH + kk);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bw, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bw[s], acc, 0, 0, 0);
     }
     if (n < f.nout) {
       const float bo = f.b[f.depth][n];
