@@ -1065,13 +1065,46 @@ struct EvalMilkyWayPieces : EvalMilkyWayTracked {
     // around each lookup's reads, serialising the lookups.
     double tc[N];
     int k[N];
+    auto piece_of = [&](double tt) {
+      const int c = (int)fmin((tt - T.t0) * T.inv_h, cmax);
+      const double2 e0 = cl[2 * c], e1 = cl[2 * c + 1];
+      return (int)e0.x + (tt >= e0.y ? 1 : 0) + (tt >= e1.x ? 1 : 0);
+    };
+    double lo = fmin(fmax(t[0], T.t0), T.t1), hi = lo;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       tc[i] = fmin(fmax(t[i], T.t0), T.t1);
-      const int c = (int)fmin((tc[i] - T.t0) * T.inv_h, cmax);
-      const double2 e0 = cl[2 * c], e1 = cl[2 * c + 1];
-      k[i] = (int)e0.x + (tc[i] >= e0.y ? 1 : 0) + (tc[i] >= e1.x ? 1 : 0);
+      lo = fmin(lo, tc[i]);
+      hi = fmax(hi, tc[i]);
     }
+    // A step is much shorter than the progenitor's: when the batch's
+    // earliest and latest times share a piece on every active lane, so do
+    // all its times, and one piece read serves them.
+    const int klo = piece_of(lo);
+    if (__all(klo == piece_of(hi))) {
+      const double2* r = pc + (kPieceDoubles / 2) * klo;
+      double c[kPieceDoubles];
+#pragma unroll
+      for (int j = 0; j < kPieceDoubles / 2; ++j) {
+        const double2 v = r[j];
+        c[2 * j] = v.x;
+        c[2 * j + 1] = v.y;
+      }
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double th = (tc[i] - c[0]) * c[1];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          double v = c[2 + 7 * m + 6];
+#pragma unroll
+          for (int e = 5; e >= 0; --e) v = fma(v, th, c[2 + 7 * m + e]);
+          d[i][m] = v;
+        }
+      }
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) k[i] = piece_of(tc[i]);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const double2* r = pc + (kPieceDoubles / 2) * k[i];
