@@ -219,6 +219,7 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
 // FP32, BF16 activations.
 constexpr int FH = 256;      // hidden width
 constexpr int FM = 64;       // points per block
+constexpr int FA = FM / 16;  // MFMA row tiles per block
 constexpr int FLD = FH + 8;  // padded LDS row (bf16 elements)
 constexpr int kMaxFusedLayers = 8;
 
@@ -271,25 +272,25 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs f) {
   for (int l = 1; l < f.depth; ++l) {
     __syncthreads();
     const __bf16* W = f.W[l];
-    f32x4 acc[4][4];
+    f32x4 acc[FA][4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < FA; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 2
     for (int k0 = 0; k0 < FH; k0 += 32) {
       const int kk = k0 + (lane >> 4) * 8;
-      bf16x8 af[4], bw[4];
+      bf16x8 af[FA], bw[4];
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int n = wave * 64 + b * 16 + (lane & 15);
         bw[b] = *reinterpret_cast<const bf16x8*>(W + n * FH + kk);
       }
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < FA; ++a)
         af[a] = *reinterpret_cast<const bf16x8*>(&act[cur][a * 16 + (lane & 15)][kk]);
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < FA; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bw[b], acc[a][b], 0, 0, 0);
@@ -300,7 +301,7 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs f) {
       const int n = wave * 64 + b * 16 + (lane & 15);
       const float bn = bias[n];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < FA; ++a)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           act[cur ^ 1][a * 16 + (lane >> 4) * 4 + j][n] = (__bf16)silu(acc[a][b][j] + bn);
@@ -308,9 +309,9 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs f) {
     cur ^= 1;
   }
   __syncthreads();
-  // ---- output layer on the matrix cores: wave w takes points
-  // w*16 .. w*16+15 against the outputs padded to 16 columns (zero weights
-  // past nout)
+  // ---- output layer on the matrix cores: wave w takes the 16-point row
+  // tiles w, w + 4, ... against the outputs padded to 16 columns (zero
+  // weights past nout)
   {
     const __bf16* Wo = f.W[f.depth];
     const int n = lane & 15;
@@ -324,19 +325,23 @@ __global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs f) {
       const u32x4 v = *reinterpret_cast<const u32x4*>(Wo + nr * FH + s * 32 + (lane >> 4) * 8);
       bw[s] = __builtin_bit_cast(bf16x8, v & mask);
     }
-    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+    // wave w takes row tiles w, w + 4, ...
 #pragma unroll
-    for (int s = 0; s < FH / 32; ++s) {
-      const int kk = s * 32 + (lane >> 4) * 8;
-      const bf16x8 af = *reinterpret_cast<const bf16x8*>(&act[cur][wave * 16 + n][kk]);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bw[s], acc, 0, 0, 0);
-    }
-    if (n < f.nout) {
-      const float bo = f.b[f.depth][n];
+    for (int a = wave; a < FA; a += 4) {
+      f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const long long m = m0 + wave * 16 + (lane >> 4) * 4 + j;
-        if (m < f.n) f.Y[m * f.ldy + n] = acc[j] + bo;
+      for (int s = 0; s < FH / 32; ++s) {
+        const int kk = s * 32 + (lane >> 4) * 8;
+        const bf16x8 af = *reinterpret_cast<const bf16x8*>(&act[cur][a * 16 + n][kk]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bw[s], acc, 0, 0, 0);
+      }
+      if (n < f.nout) {
+        const float bo = f.b[f.depth][n];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const long long m = m0 + a * 16 + (lane >> 4) * 4 + j;
+          if (m < f.n) f.Y[m * f.ldy + n] = acc[j] + bo;
+        }
       }
     }
   }
