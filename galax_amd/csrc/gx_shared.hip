@@ -9,7 +9,8 @@
 // (legacy/funcs.py:42-213).  This file reproduces those semantics on the
 // GPU; gx_integrate.hip gives every orbit its own steps instead.
 //
-// Design: one kernel launch per attempted step.  Each lane evaluates the
+// Design: one attempted step per launch, or per trip of one cooperative
+// launch when the batch's workgroups all fit on the device at once.  Each lane evaluates the
 // method for one orbit (the same code as the independent integrator,
 // gx_rk.hpp); each workgroup reduces its error sum into a partial; the last
 // workgroup to finish (a device-scope ticket) reduces the partials in a fixed
@@ -24,6 +25,8 @@
 // early stop marks every save past the last accepted step NaN.
 // The host launches steps in batches and reads the record's `done` flag
 // between batches; launches after the end return at once.
+#include <hip/hip_cooperative_groups.h>
+
 #include <cmath>
 
 #include "gx_internal.hpp"
@@ -65,7 +68,8 @@ struct SArgs {
   long long* naccept;
   SCtl* ctl;
   unsigned int* counter;
-  double* part;  // two partial sums per workgroup
+  double* part;  // two partial sums per workgroup, in two sets (the
+                 // cooperative solve alternates them by trip)
   double* y;     // two state buffers of n x 9: q, p (solver variables), and
                  // a(q, tau), the step start's first stage
   double* rec;   // dense records [n][rec_cap][kDenseRec], or null
@@ -278,18 +282,21 @@ __device__ __forceinline__ double clip_end(double tprev, double tnext,
                               : tnext;
 }
 
-// One attempted step of the whole batch, with the save times inside it
-// written from the step's dense output (before the decision; see above).
+// One attempted step of the whole batch for this lane's orbit, with the
+// save times inside it written from the step's dense output (before the
+// decision; see above): the lane's squared scaled error and a non-finite
+// flag for the batch's norm.
 template <class M, bool PID, class E>
-__global__ void __launch_bounds__(kShBlock) k_sh_step(DPot P, SArgs A) {
-  const SCtl c = *A.ctl;
-  if (c.done) return;
+__device__ __forceinline__ void sh_attempt(const DPot& P, const SArgs& A,
+                                           const SCtl& c, double& sq,
+                                           double& bad) {
   const long long i = (long long)blockIdx.x * kShBlock + threadIdx.x;
   const bool valid = i < A.n;
   const double dir = A.dir;
   const double* ycur = A.y + (long long)c.cur * kRow * A.n;
   double* yoth = A.y + (long long)(c.cur ^ 1) * kRow * A.n;
-  double sq = 0.0, bad = 0.0;
+  sq = 0.0;
+  bad = 0.0;
   if (valid && !c.finished) {
     constexpr int SF = M::SF;
     const double h = c.tnext - c.tau;
@@ -326,70 +333,132 @@ __global__ void __launch_bounds__(kShBlock) k_sh_step(DPot P, SArgs A) {
       }
     }
   }
-  double SQ, NB;
-  if (!last_block_sums(A, sq, bad, SQ, NB)) return;
-  if (threadIdx.x != 0) return;
+}
+
+// The controller for the whole batch: the record after the attempt that
+// started from `c`, given the batch's summed squared scaled error SQ and
+// non-finite count NB.
+template <bool PID>
+__device__ SCtl sh_control(const SArgs& A, const SCtl& c, double SQ, double NB) {
   SCtl n = c;
   if (c.finished) {
     n.done = 1;
-  } else {
-    n.nsteps = c.nsteps + 1;
-    const double h = c.tnext - c.tau;
-    if (NB > 0.0 || !(h >= 0.0)) {
-      n.status = GX_NONFINITE;
-      n.finished = 1;
-    } else {
-      bool keep = true;
-      double next_t1;
-      if (PID) {
-        const double scaled = sqrt(SQ / (6.0 * (double)A.n));
-        keep = scaled < 1.0;
-        if (A.dtmin > 0.0) keep = keep || c.at_dtmin;
-        const double inv = 1.0 / scaled;
-        const double f1 = (A.c1 == 0.0) ? 1.0 : pow(inv, A.c1);
-        const double f2 = (A.c2 == 0.0) ? 1.0 : pow(c.prev_inv, A.c2);
-        const double f3 = (A.c3 == 0.0) ? 1.0 : pow(c.prev_prev_inv, A.c3);
-        const double fmin_ = keep ? 1.0 : A.factormin;
-        const double factor =
-            fmin(fmax(A.safety * f1 * f2 * f3, fmin_), A.factormax);
-        double dt = fmin(h * factor, A.dtmax);
-        if (A.dtmin > 0.0) {
-          if (!A.force_dtmin && dt < A.dtmin) n.status = GX_DT_MIN_REACHED;
-          n.at_dtmin = dt <= A.dtmin ? 1 : 0;
-          dt = fmax(dt, A.dtmin);
-        }
-        if (keep) {
-          n.prev_prev_inv = c.prev_inv;
-          n.prev_inv = (inv == 0.0 || isinf(inv)) ? 1.0 : inv;
-        }
-        next_t1 = (keep ? c.tnext : c.tau) + dt;
-      } else {
-        next_t1 = c.tnext + A.dt0;
-      }
-      if (keep) {
-        n.cur = c.cur ^ 1;
-        n.naccept = c.naccept + 1;
-        if (A.rec && n.naccept > A.rec_cap) {
-          n.status = GX_MAX_STEPS_REACHED;  // out of record slots
-          n.finished = 1;
-        }
-        int s = c.si;
-        while (s < A.T && dir * A.ts[s] <= c.tnext) ++s;
-        n.si = s;
-      }
-      const double tprev = fmin(keep ? c.tnext : c.tau, c.tau1);
-      n.tnext = clip_end(tprev, next_t1, c.tau1, keep);
-      n.tau = tprev;
-      if (n.status == GX_DT_MIN_REACHED || n.tau >= c.tau1)
-        n.finished = 1;
-      else if (n.nsteps >= A.max_steps) {
-        n.status = GX_MAX_STEPS_REACHED;
-        n.finished = 1;
-      }
-    }
+    return n;
   }
-  *A.ctl = n;
+  const double dir = A.dir;
+  n.nsteps = c.nsteps + 1;
+  const double h = c.tnext - c.tau;
+  if (NB > 0.0 || !(h >= 0.0)) {
+    n.status = GX_NONFINITE;
+    n.finished = 1;
+    return n;
+  }
+  bool keep = true;
+  double next_t1;
+  if (PID) {
+    const double scaled = sqrt(SQ / (6.0 * (double)A.n));
+    keep = scaled < 1.0;
+    if (A.dtmin > 0.0) keep = keep || c.at_dtmin;
+    const double inv = 1.0 / scaled;
+    const double f1 = (A.c1 == 0.0) ? 1.0 : pow(inv, A.c1);
+    const double f2 = (A.c2 == 0.0) ? 1.0 : pow(c.prev_inv, A.c2);
+    const double f3 = (A.c3 == 0.0) ? 1.0 : pow(c.prev_prev_inv, A.c3);
+    const double fmin_ = keep ? 1.0 : A.factormin;
+    const double factor = fmin(fmax(A.safety * f1 * f2 * f3, fmin_), A.factormax);
+    double dt = fmin(h * factor, A.dtmax);
+    if (A.dtmin > 0.0) {
+      if (!A.force_dtmin && dt < A.dtmin) n.status = GX_DT_MIN_REACHED;
+      n.at_dtmin = dt <= A.dtmin ? 1 : 0;
+      dt = fmax(dt, A.dtmin);
+    }
+    if (keep) {
+      n.prev_prev_inv = c.prev_inv;
+      n.prev_inv = (inv == 0.0 || isinf(inv)) ? 1.0 : inv;
+    }
+    next_t1 = (keep ? c.tnext : c.tau) + dt;
+  } else {
+    next_t1 = c.tnext + A.dt0;
+  }
+  if (keep) {
+    n.cur = c.cur ^ 1;
+    n.naccept = c.naccept + 1;
+    if (A.rec && n.naccept > A.rec_cap) {
+      n.status = GX_MAX_STEPS_REACHED;  // out of record slots
+      n.finished = 1;
+    }
+    int s = c.si;
+    while (s < A.T && dir * A.ts[s] <= c.tnext) ++s;
+    n.si = s;
+  }
+  const double tprev = fmin(keep ? c.tnext : c.tau, c.tau1);
+  n.tnext = clip_end(tprev, next_t1, c.tau1, keep);
+  n.tau = tprev;
+  if (n.status == GX_DT_MIN_REACHED || n.tau >= c.tau1)
+    n.finished = 1;
+  else if (n.nsteps >= A.max_steps) {
+    n.status = GX_MAX_STEPS_REACHED;
+    n.finished = 1;
+  }
+  return n;
+}
+
+// One attempted step of the whole batch per launch: the last workgroup to
+// publish its partial sums runs the controller and writes the record.
+template <class M, bool PID, class E>
+__global__ void __launch_bounds__(kShBlock) k_sh_step(DPot P, SArgs A) {
+  const SCtl c = *A.ctl;
+  if (c.done) return;
+  double sq, bad;
+  sh_attempt<M, PID, E>(P, A, c, sq, bad);
+  double SQ, NB;
+  if (!last_block_sums(A, sq, bad, SQ, NB)) return;
+  if (threadIdx.x != 0) return;
+  *A.ctl = sh_control<PID>(A, c, SQ, NB);
   *A.counter = 0u;
+}
+
+// The whole solve in one cooperative launch (every workgroup resident):
+// each trip is one attempted step; the workgroups publish their partial
+// sums (double-buffered by trip parity, so that a workgroup already on the
+// next trip cannot overwrite sums another is still reading), meet at one
+// grid barrier, and then every workgroup sums the partials in index order
+// -- the same order and arithmetic as last_block_sums -- and runs the
+// controller itself, so all take the same decision without a second
+// barrier.  Workgroup 0 also writes the record for the host and the finish
+// kernel.  Every workgroup leaves on the same trip (the record's done).
+template <class M, bool PID, class E>
+__global__ void __launch_bounds__(kShBlock) k_sh_persist(DPot P, SArgs A) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  __shared__ SCtl cs;
+  __shared__ double lds[kShWaves];
+  if (threadIdx.x == 0) cs = *A.ctl;
+  __syncthreads();
+  for (long long trip = 0;; ++trip) {
+    const SCtl c = cs;
+    if (c.done) break;
+    double sq, bad;
+    sh_attempt<M, PID, E>(P, A, c, sq, bad);
+    const double a = block_sum(sq, lds), b = block_sum(bad, lds);
+    double* part = A.part + 2 * (long long)gridDim.x * (trip & 1);
+    if (threadIdx.x == 0) {
+      part[2 * blockIdx.x] = a;
+      part[2 * blockIdx.x + 1] = b;
+      __threadfence();
+    }
+    grid.sync();
+    double x = 0.0, z = 0.0;
+    for (unsigned int k = threadIdx.x; k < gridDim.x; k += kShBlock) {
+      x += __hip_atomic_load(&part[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      z += __hip_atomic_load(&part[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const double SQ = block_sum(x, lds), NB = block_sum(z, lds);
+    if (threadIdx.x == 0) {
+      const SCtl n = sh_control<PID>(A, c, SQ, NB);
+      cs = n;
+      if (blockIdx.x == 0) *A.ctl = n;
+    }
+    __syncthreads();
+  }
 }
 
 // Per-orbit results: NaN for save times never reached; the batch's status
@@ -446,6 +515,32 @@ int run(const gx_pot* pot, SArgs& A, bool first_step, hipStream_t stream) {
                        stream, pot->dev, A, (double)M::order);
   }
   if (hipGetLastError() != hipSuccess) return GX_ELAUNCH;
+  // The whole solve in one cooperative launch when every workgroup of the
+  // batch can be resident at once (the launch is refused otherwise, and the
+  // step launches below take over).  The call stays synchronous.
+  {
+    const auto pk = k_sh_persist<M, PID, E>;
+    int coop = 0, occ = 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch,
+                              pot->device) == hipSuccess &&
+        coop &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pk, kShBlock, 0) ==
+            hipSuccess &&
+        (long long)blocks <= (long long)occ * num_cus(pot->device)) {
+      DPot Pd = pot->dev;
+      SArgs Ad = A;
+      void* args[] = {&Pd, &Ad};
+      if (hipLaunchCooperativeKernel(reinterpret_cast<const void*>(pk),
+                                     dim3(blocks), dim3(kShBlock), args, 0,
+                                     stream) == hipSuccess) {
+        hipLaunchKernelGGL(k_sh_finish, dim3(blocks), dim3(kShBlock), 0,
+                           stream, A);
+        if (hipGetLastError() != hipSuccess) return GX_ELAUNCH;
+        return hipStreamSynchronize(stream) == hipSuccess ? GX_OK : GX_ELAUNCH;
+      }
+      (void)hipGetLastError();
+    }
+  }
   // Step launches go out in batches; after each batch the record's `done`
   // flag is copied to pinned host memory behind an event.  Two batches are
   // in flight, so the GPU never waits for the host's check.  Every launch
@@ -534,7 +629,7 @@ extern "C" {
 int64_t gx_integrate_shared_workspace_bytes(int64_t n) {
   if (n < 0) return 0;
   const int64_t blocks = (n + kShBlock - 1) / kShBlock;
-  return 512 + 16 * blocks + 8 * (2 * kRow * n);
+  return 512 + 32 * blocks + 8 * (2 * kRow * n);
 }
 
 }  // extern "C"
@@ -593,8 +688,8 @@ int shared_solve(const gx_pot* pot, const gx_solver_cfg* cfg,
   const int64_t blocks = (n + kShBlock - 1) / kShBlock;
   A.ctl = reinterpret_cast<SCtl*>(ws);
   A.counter = reinterpret_cast<unsigned int*>(ws + 256);
-  A.part = reinterpret_cast<double*>(ws + 512);
-  A.y = A.part + 2 * blocks;
+  A.part = reinterpret_cast<double*>(ws + 512);  // two sets of partials
+  A.y = A.part + 4 * blocks;
   A.rec = rec;
   A.rec_cap = rec_cap;
   A.nrec = reinterpret_cast<long long*>(nrec);

@@ -333,9 +333,11 @@ int gx_integrate_shared_dense(const gx_pot* pot, const gx_solver_cfg* cfg,
  * status[n], nsteps[n], naccept[n] (optional) receive the batch's values for
  * every orbit.  `workspace`: gx_integrate_shared_workspace_bytes(n) bytes of
  * device memory (256-byte aligned) that no concurrent call uses.
- * Synchronous: the host launches one kernel per attempted step, in batches,
- * and returns when the solve has ended (it reads a 4-byte flag after each
- * batch); the results are then on `stream`. */
+ * Synchronous: when every workgroup of the batch can be resident at once,
+ * one cooperative launch runs the whole solve (a grid barrier per attempted
+ * step); otherwise the host launches one kernel per attempted step, in
+ * batches, reading a 4-byte flag after each batch.  Either way the call
+ * returns when the solve has ended; the results are then on `stream`. */
 int64_t gx_integrate_shared_workspace_bytes(int64_t n);
 int gx_integrate_shared(const gx_pot* pot, const gx_solver_cfg* cfg,
                         const double* w0, double t0, double t1,

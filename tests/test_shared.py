@@ -47,7 +47,7 @@ def test_shared_entry_point_exported():
     ws = _lib.lib().gx_integrate_shared_workspace_bytes
     # control record + ticket + two partials per 256-orbit workgroup + two
     # state buffers of (q, p, step-start acceleration)
-    assert ws(1000) == 512 + 16 * 4 + 8 * 18 * 1000
+    assert ws(1000) == 512 + 32 * 4 + 8 * 18 * 1000
     assert ws(-1) == 0
 
 
