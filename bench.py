@@ -151,7 +151,7 @@ def stream_report(steps: int) -> dict:
     gd.integrate_orbits(field, w_all[:256], t0[:256], t1[:256], solver=solver)
     torch.cuda.synchronize()
     # each pass timed on its own: one particle's serial steps bound a pass,
-    # and passes of the same work spread by a quarter on one box
+    # and passes of the same work on one device spread by a quarter
     passes = []
     for _ in range(max(steps, 1)):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
