@@ -40,7 +40,7 @@ HIPCC_FLAGS = [
 # spills them to vector lanes, reloading them on every trip.  Without it the
 # Milky Way kernel needs 181 VGPRs instead of 250 and half the lane reads,
 # and the throughput pass measured 19.1-19.4 ms instead of 20.1-20.5 ms
-# (alternating runs on two boxes, same step counts).
+# (alternating runs on two devices, same step counts).
 SOURCE_FLAGS = {"gx_integrate.hip": ["-mllvm", "-disable-machine-licm"]}
 
 
