@@ -59,8 +59,7 @@ struct AArgs {
   int ntheta;
   double* ckpt;  // [C][7][n]
   double* seg;   // [K][7][n]
-  double* stq;   // [K][SF][3][n]: per step of a segment, its stage positions
-                 // and end position q1
+  double* stq;   // [SF][3][n]: stage positions and the end position q1
   int* status;
 };
 
@@ -83,12 +82,11 @@ __device__ __forceinline__ double clip_end(double tprev, double tnext,
   return (tnext > t1 - 1e-10) ? t1 : tnext;
 }
 
-// Forward step (no error estimate); optionally records the stage positions
-// at stq (the step's [SF][3][n] block).
+// Forward step (no error estimate); optionally records the stage positions.
 template <class M, class E, bool REC>
 __device__ __forceinline__ void step_fwd(const DPot& P, const AArgs& A,
                                          long long i, double h, double (&q)[3],
-                                         double (&p)[3], double* stq = nullptr) {
+                                         double (&p)[3]) {
   constexpr int S = M::S;
   double g[S][3];
 #pragma unroll
@@ -116,7 +114,7 @@ __device__ __forceinline__ void step_fwd(const DPot& P, const AArgs& A,
     }
     if (REC) {
 #pragma unroll
-      for (int m = 0; m < 3; ++m) stq[((long long)k * 3 + m) * A.n + i] = Q[m];
+      for (int m = 0; m < 3; ++m) A.stq[((long long)k * 3 + m) * A.n + i] = Q[m];
     }
     E::acc(P, Q[0], Q[1], Q[2], 0.0, g[k][0], g[k][1], g[k][2]);
   }
@@ -139,9 +137,9 @@ __device__ __forceinline__ void step_fwd(const DPot& P, const AArgs& A,
   const double q1 = fma(h, fma(sb, p[1], h * c1), q[1]);
   const double q2 = fma(h, fma(sb, p[2], h * c2), q[2]);
   if (REC && M::SF > S) {
-    stq[((long long)S * 3 + 0) * A.n + i] = q0;
-    stq[((long long)S * 3 + 1) * A.n + i] = q1;
-    stq[((long long)S * 3 + 2) * A.n + i] = q2;
+    A.stq[((long long)S * 3 + 0) * A.n + i] = q0;
+    A.stq[((long long)S * 3 + 1) * A.n + i] = q1;
+    A.stq[((long long)S * 3 + 2) * A.n + i] = q2;
   }
   p[0] = fma(h, b0, p[0]);
   p[1] = fma(h, b1, p[1]);
@@ -176,9 +174,13 @@ __device__ __forceinline__ void step_adj(const DPot& P, const AArgs& A,
                                          const double (&gx)[M::SF][3],
                                          const double (&dq)[3],
                                          const double (&dp)[3],
-                                         bool any_save, const double* stq) {
+                                         bool any_save) {
   constexpr int S = M::S;
   constexpr int SF = M::SF;
+  {
+    double qq[3] = {q[0], q[1], q[2]}, pp[3] = {p[0], p[1], p[2]};
+    step_fwd<M, E, true>(P, A, i, h, qq, pp);
+  }
   double gb[SF][3];
   const double h2 = h * h;
 #pragma unroll
@@ -197,7 +199,7 @@ __device__ __forceinline__ void step_adj(const DPot& P, const AArgs& A,
       // the end stage g_E = a(q1), q1 = q + h (sb p + h sum (bA)_j g_j)
       double Q[3], Hw[3];
 #pragma unroll
-      for (int m = 0; m < 3; ++m) Q[m] = stq[((long long)S * 3 + m) * A.n + i];
+      for (int m = 0; m < 3; ++m) Q[m] = A.stq[((long long)S * 3 + m) * A.n + i];
       adj_all<K...>(P, Q, gb[S], Hw, th, std::make_index_sequence<sizeof...(K)>{});
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
@@ -218,7 +220,7 @@ __device__ __forceinline__ void step_adj(const DPot& P, const AArgs& A,
   for (int k = S - 1; k >= 0; --k) {
     double Q[3], Hw[3];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) Q[m] = stq[((long long)k * 3 + m) * A.n + i];
+    for (int m = 0; m < 3; ++m) Q[m] = A.stq[((long long)k * 3 + m) * A.n + i];
     adj_all<K...>(P, Q, gb[k], Hw, th, std::make_index_sequence<sizeof...(K)>{});
     // Qb_k = -H gb_k
     const double rs = M::k.rs[k];
@@ -315,11 +317,12 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(DPot P, AArgs A) {
         sg[4 * A.n] = ps[1];
         sg[5 * A.n] = ps[2];
         sg[6 * A.n] = ts_;
-        const double tn = (n0 + m == 0) ? fmin(ts_ + A.dt0, tau1)
-                                        : clip_end(ts_, ts_ + A.dt0, tau1);
-        step_fwd<M, E, true>(P, A, i, tn - ts_, qs, ps,
-                             A.stq + (long long)m * M::SF * 3 * A.n);
-        ts_ = tn;
+        if (m + 1 < len) {
+          const double tn = (n0 + m == 0) ? fmin(ts_ + A.dt0, tau1)
+                                          : clip_end(ts_, ts_ + A.dt0, tau1);
+          step_fwd<M, E, false>(P, A, i, tn - ts_, qs, ps);
+          ts_ = tn;
+        }
       }
       // reverse over the segment's steps
       for (int m = len - 1; m >= 0; --m) {
@@ -367,8 +370,7 @@ __global__ void __launch_bounds__(kAdjBlock) k_adjoint(DPot P, AArgs A) {
           any_save = true;
           --si;
         }
-        step_adj<M, E, K...>(P, A, i, h, qn, pn, qb, pb, th, gx, dq, dp, any_save,
-                             A.stq + (long long)m * M::SF * 3 * A.n);
+        step_adj<M, E, K...>(P, A, i, h, qn, pn, qb, pb, th, gx, dq, dp, any_save);
       }
     }
     // saves at (or before) the start
@@ -436,7 +438,7 @@ extern "C" {
 int64_t gx_adjoint_workspace_bytes(int64_t n, int64_t max_steps, int32_t K) {
   if (n < 0 || max_steps < 0 || K < 1) return 0;
   const int64_t C = max_steps / K + 2;
-  return 8 * n * (7 * C + 7 * (int64_t)K + 3 * 14 * (int64_t)K) + 256;
+  return 8 * n * (7 * C + 7 * (int64_t)K + 3 * 14) + 256;
 }
 
 int gx_adjoint(const gx_pot* pot, const gx_solver_cfg* cfg, const double* w0,
